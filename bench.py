@@ -1,0 +1,185 @@
+"""Benchmark: backpropagate_3d Rytov reconstruction, 256x256 detector,
+360 angles (padded to 512x512), 256^3 output, on one MI355X.
+
+A step is one full backpropagate_3d call (pad, FFT, filter, per-z
+propagation and inverse transform, fused prefilter, rotate-accumulate over
+all 360 angles) on a complex64 sinogram already resident in HBM, computed
+in float32.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("voxel-angle updates/s (256² det, 360 angles) FP32 vs FP64 GPU "
+          "at ≤1e-5 rel L2")
+N_DET, N_ANG = 256, 360
+RES, NM = 6.0, 1.333
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per rotate-accumulate launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), read
+    from the committed summary if present."""
+    f = ROOT / "profiles" / "rotate_acc_c2_traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(uB, angles, seconds):
+    """Oracle (CPU restatement of the reference, float64, y-chunk process
+    pool + threaded FFTs) on a bounded, contiguous sample of the angles;
+    per-angle cost extrapolated linearly to all angles plus the setup."""
+    from oracle import odt_oracle
+    cores = min(16, os.cpu_count() or 1)
+    stats = {}
+    odt_oracle.backpropagate_3d(uB, angles, RES, NM, padval="edge",
+                                intp_order=2, num_cores=cores,
+                                time_limit=seconds, stats=stats)
+    per_angle = stats["loop_s"] / stats["angles"]
+    total = stats["setup_s"] + per_angle * len(angles)
+    V = N_DET ** 3
+    return {"value": V * len(angles) / total, "unit": "voxel-angle updates/s",
+            "cores": cores, "kind": "port",
+            "sample": (f"oracle float64, {stats['angles']} of {len(angles)} "
+                       f"angles ({stats['loop_s']:.1f} s) + setup "
+                       f"{stats['setup_s']:.1f} s, extrapolated: "
+                       f"{total:.1f} s per reconstruction")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; "
+                          "multi-GPU angle sharding is a design (DESIGN.md) "
+                          "and is not benchmarked"}))
+        sys.exit(2)
+
+    import torch
+    import odtbrain_amd
+    from odtbrain_amd import _core, phantoms
+
+    dev = torch.device("cuda:0")
+    log("bench: generating 256^2 x 360 ellipsoid Rytov sinogram")
+    uB, angles = phantoms.ellipsoid_sinogram_3d(N=N_DET, A=N_ANG)
+    d_in = torch.from_numpy(uB).to(dev)
+    kw = dict(res=RES, nm=NM, padval="edge", intp_order=2)
+
+    def step(dtype):
+        return odtbrain_amd.backpropagate_3d(d_in, angles, dtype=dtype, **kw)
+
+    log("bench: warmup")
+    for _ in range(args.warmup):
+        out32 = step(np.float32)
+    torch.cuda.synchronize()
+    log("bench: timed steps fp32")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out32 = step(np.float32)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    V = N_DET ** 3
+    value = V * N_ANG / t_step
+
+    # rotate-accumulate kernel time from HIP events on the plan's stream
+    plan = _core.last_plan
+    plan.enable_timing(True)
+    rot, tot = [], []
+    for _ in range(3):
+        step(np.float32)
+        tm = plan.last_timing()
+        rot.append(tm["rotate_ms"] / tm["rotate_launches"])
+        tot.append(tm["total_ms"])
+    plan.enable_timing(False)
+    rot_ms = float(np.mean(rot))
+    launches = tm["rotate_launches"]
+    ang_per_launch = N_ANG / launches
+    alg_bytes = ang_per_launch * V * 8 + V * 8
+    achieved = alg_bytes / (rot_ms * 1e-3) / 1e9
+    log(f"bench: fp32 {t_step * 1e3:.1f} ms/step, rotate {rot_ms:.2f} ms "
+        f"x{launches}, plan total {np.mean(tot):.1f} ms")
+
+    fp64 = None
+    if not args.no_fp64:
+        log("bench: fp64 path")
+        out64 = step(np.float64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n64 = max(1, args.steps // 2)
+        for _ in range(n64):
+            out64 = step(np.float64)
+        torch.cuda.synchronize()
+        t64 = (time.perf_counter() - t0) / n64
+        a = out32.to(torch.complex128)
+        err = float(torch.linalg.vector_norm(a - out64) /
+                    torch.linalg.vector_norm(out64))
+        maxabs = float((a - out64).abs().max())
+        fp64 = {"value": V * N_ANG / t64, "ms_per_step": t64 * 1e3,
+                "fp32_rel_l2_vs_fp64": err, "fp32_max_abs_vs_fp64": maxabs}
+        log(f"bench: fp64 {t64 * 1e3:.1f} ms/step, fp32 vs fp64 rel_l2 "
+            f"{err:.2e}")
+        del out64
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline (oracle, bounded sample)")
+        cpu = cpu_baseline(uB, angles, args.cpu_seconds)
+        log(f"bench: cpu {cpu['value']:.3e} voxel-angle/s ({cpu['sample']})")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-angle updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 3-ellipsoid Rytov phantom, complex64)",
+        "config": {"workload": "backpropagate_3d Rytov, 256x256 detector, "
+                               "360 angles uniform on [0,2pi), padded to "
+                               "512x512, output 256^3 complex64",
+                   "res": RES, "nm": NM, "intp_order": 2, "padval": "edge"},
+        "roofline": {"bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(),
+                     "kernel": "k_rotate_acc", "ms_per_launch": rot_ms,
+                     "angles_per_launch": ang_per_launch},
+        "cpu_baseline": cpu,
+        "fp64_gpu": fp64,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,158 @@
+/*
+ * odtbrain_hip.h -- C ABI of the gfx950 filtered-backpropagation library
+ * (libodtbrain_hip.so).
+ *
+ * The reference (ODTbrain 0.4.12) is pure Python; its hot path is the body
+ * of three functions whose arithmetic runs in third-party native code
+ * (FFTW via pyfftw, numexpr, scipy.ndimage).  This library replaces that
+ * body.  The Python package odtbrain_amd keeps the reference signatures
+ * and validation and binds the entry points below with ctypes:
+ *
+ *   odt_plan_create / odt_plan_execute / odt_plan_destroy
+ *     replace the per-call body of
+ *       backpropagate_3d        odtbrain/_alg3d_bpp.py:254-555
+ *       backpropagate_3d_tilted odtbrain/_alg3d_bppt.py:581-949
+ *       backpropagate_2d        odtbrain/_alg2d_bpp.py:142-326
+ *     (pad -> FFT2 -> prefactor -> per-z propagator x IFFT2 -> crop ->
+ *      spline prefilter -> rotate/affine resample -> sum over angles)
+ *   odt_fft_c2c            replaces the pyfftw.FFTW plans
+ *                          (_alg3d_bpp.py:460-478) and numpy.fft (_alg2d_bpp.py:264,306)
+ *   odt_spline_prefilter   replaces scipy.ndimage.spline_filter(1d) as called
+ *                          inside scipy.ndimage.rotate/affine_transform
+ *                          (_alg3d_bpp.py:73-81, _alg3d_bppt.py:931-944)
+ *   odt_rotate_accumulate  replaces _mprotate/_rotate + ``outarr += arr``
+ *                          (_alg3d_bpp.py:35-81, 532-546) on prefiltered planes
+ *
+ * Conventions: every function returns 0 on success and a nonzero ODT_ERR_*
+ * code otherwise; odt_last_error() returns a thread-local message.  No C++
+ * exception crosses the ABI.  All data pointers named d_* are device
+ * pointers owned by the caller; host pointers are read during the call only.
+ * A plan owns its small device tables; the large workspace is passed in by
+ * the caller (odt_plan_workspace_size).  A plan is thread-compatible, not
+ * thread-safe: use one plan per stream.
+ */
+#ifndef ODTBRAIN_HIP_H
+#define ODTBRAIN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ODT_OK 0
+#define ODT_ERR_INVALID 1     /* bad argument (shape, size, null pointer) */
+#define ODT_ERR_HIP 2         /* HIP runtime error (message has details) */
+#define ODT_ERR_UNSUPPORTED 3 /* valid but not implemented combination */
+#define ODT_ERR_WORKSPACE 4   /* workspace too small */
+
+#define ODT_KIND_3D 0         /* backpropagate_3d          (_alg3d_bpp.py:84)  */
+#define ODT_KIND_3D_TILTED 1  /* backpropagate_3d_tilted   (_alg3d_bppt.py:374) */
+#define ODT_KIND_2D 2         /* backpropagate_2d          (_alg2d_bpp.py:8)   */
+
+#define ODT_FP32 0            /* complex64 data, float32 arithmetic */
+#define ODT_FP64 1            /* complex128 data, float64 arithmetic */
+
+#define ODT_PAD_NONE 0        /* no padding          (_alg3d_bpp.py:493-495) */
+#define ODT_PAD_EDGE 1        /* np.pad mode="edge"  (_alg3d_bpp.py:496-500) */
+#define ODT_PAD_RAMP 2        /* np.pad mode="linear_ramp", end_values=padval
+                                 (_alg3d_bpp.py:501-506) */
+
+/* flags */
+#define ODT_FLAG_IIR_PREFILTER 1 /* run the separable IIR prefilter kernel on
+                                    the slabs instead of folding the prefilter
+                                    into the inverse transforms */
+
+/* Plan description.  Geometry is computed by the caller in float64 exactly as
+ * the reference computes it (pad sizes _alg3d_bpp.py:285-304; rotation
+ * matrices of scipy.ndimage.rotate / affine_transform). */
+typedef struct odt_desc {
+  int32_t kind;          /* ODT_KIND_* */
+  int32_t precision;     /* ODT_FP32 / ODT_FP64 */
+  int32_t A;             /* number of projections */
+  int32_t Ny, Nx;        /* detector rows/cols (2D: Ny = 1, Nx = N) */
+  int32_t lNy, lNx;      /* transform sizes after padding */
+  int32_t padyl, padxl;  /* leading pad widths */
+  int32_t pad_mode;      /* ODT_PAD_* */
+  int32_t intp_order;    /* spline order 0..5 */
+  int32_t onlyreal;      /* nonzero: real part only */
+  int32_t flags;         /* ODT_FLAG_* */
+  int32_t angle_param_stride; /* doubles per angle in angle_params: 6 or 12 */
+  double padval_re, padval_im; /* linear-ramp end value */
+  double km;             /* 2*pi*nm/res */
+  double lD;             /* detector distance */
+  double z0;             /* z of slice 0: -ln/2 (3D) or -ln/2+0.5 (2D) */
+  double ramp_u, ramp_v; /* ramp |kx*v + ky*u|; u=0, v=1 for the y axis */
+  const double *weights; /* host, A entries: angle weights (1 if unweighted) */
+  /* host, A*angle_param_stride entries.
+   * planar (stride 6): m00 m01 m10 m11 off0 off1, input = M @ (o_z, o_x) + off
+   *   (the matrix and offset of scipy.ndimage.rotate, axes (z, x));
+   * affine (stride 12): 3x3 row-major Dinv then offset, input index
+   *   = Dinv @ (x, Ny-1-y, z) + offset in the (x, -y, z) frame of
+   *   _alg3d_bppt.py:904-936. */
+  const double *angle_params;
+  int64_t slab_budget_bytes; /* upper bound for the batch of filtered slabs
+                                (0: library default) */
+} odt_desc;
+
+typedef struct odt_plan odt_plan;
+
+/* Build a plan: validates the description, derives the transform support,
+ * uploads twiddles, filter and geometry tables.  Allocates device memory for
+ * the tables only. */
+int odt_plan_create(const odt_desc *desc, odt_plan **plan);
+
+/* Bytes of caller-provided device workspace that odt_plan_execute needs. */
+int odt_plan_workspace_size(const odt_plan *plan, size_t *bytes);
+
+/* Run the full reconstruction on `stream` (a hipStream_t, 0 = null stream).
+ * d_usin: (A, Ny, Nx) complex64 (FP32) or complex128 (FP64), C order.
+ * d_out:  (Nx, Ny, Nx) complex, or real if onlyreal; overwritten.
+ * The call is asynchronous with respect to the host. */
+int odt_plan_execute(odt_plan *plan, const void *d_usin, void *d_out,
+                     void *d_workspace, size_t workspace_bytes, void *stream);
+
+int odt_plan_destroy(odt_plan *plan);
+
+/* Kernel timing of the last execute (milliseconds, HIP events on the plan's
+ * stream), for the rotate-accumulate kernel and the whole call.  Timing is
+ * recorded only when enabled. */
+int odt_plan_enable_timing(odt_plan *plan, int enable);
+int odt_plan_last_timing(const odt_plan *plan, double *rotate_ms,
+                         double *filter_ms, double *total_ms,
+                         int *rotate_launches);
+
+/* Batched 1D complex transforms along the last axis, unnormalised.
+ * direction -1: forward (exp(-2 pi i jk/n)), +1: backward.  n arbitrary
+ * (power of two: Stockham; otherwise direct DFT).  In place allowed. */
+int odt_fft_c2c(int precision, int64_t batch, int32_t n, int32_t direction,
+                const void *d_in, void *d_out, void *stream);
+
+/* Mirror-boundary B-spline prefilter (order 2..5) of `nlines` lines of length
+ * n, element (line l, position i) at d_data[l*line_stride + i*elem_stride]
+ * (strides in elements of the real type), in place, computed in float64
+ * internally for ODT_FP64 and float32 for ODT_FP32. */
+int odt_spline_prefilter(int precision, int order, int64_t nlines,
+                         int32_t n, int64_t line_stride, int64_t elem_stride,
+                         void *d_data, void *stream);
+
+/* Rotate-accumulate on prefiltered real planes:
+ * d_coef: (nang, ny, nz, nx) coefficient planes (plane a,y has shape nz x nx),
+ * d_out:  (nz, ny, nx), d_out[z,y,x] += sum_a interp(coef[a,y], M_a @ (z,x) + off_a)
+ * with scipy.ndimage's constant-mode rule (outside [0,n-1] contributes 0).
+ * params: host, nang*6 doubles (m00 m01 m10 m11 off0 off1).
+ * accumulate: 0 overwrites d_out, 1 adds to it. */
+int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
+                          int32_t nz, int32_t nx, const void *d_coef,
+                          const double *params, void *d_out, int accumulate,
+                          void *stream);
+
+const char *odt_last_error(void);
+int odt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ODTBRAIN_HIP_H */

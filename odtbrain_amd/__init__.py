@@ -1,0 +1,21 @@
+"""odtbrain_amd -- filtered backpropagation for optical diffraction
+tomography on AMD Instinct MI355X (gfx950).
+
+Drop-in GPU implementations of ODTbrain's backpropagation functions with the
+reference signatures (odtbrain/__init__.py:3-8):
+
+    backpropagate_3d, backpropagate_3d_tilted, backpropagate_2d
+
+The arithmetic runs in hand-written HIP kernels (libodtbrain_hip.so, C ABI
+in include/odtbrain_hip.h) called through ctypes; the host side only
+validates arguments and plans float64 geometry.
+"""
+from ._alg2d_bpp import backpropagate_2d
+from ._alg3d_bpp import backpropagate_3d
+from ._alg3d_bppt import backpropagate_3d_tilted
+from . import util
+
+__version__ = "0.1.0"
+
+__all__ = ["backpropagate_2d", "backpropagate_3d", "backpropagate_3d_tilted",
+           "util"]

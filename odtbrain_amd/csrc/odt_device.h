@@ -1,0 +1,232 @@
+// Device building blocks for the gfx950 filtered-backpropagation kernels:
+// complex arithmetic, workgroup-cooperative LDS transforms (Stockham radix-2
+// for powers of two, direct DFT otherwise), B-spline tap weights and the
+// mirror index rule of scipy.ndimage (the resampler the reference calls in
+// odtbrain/_alg3d_bpp.py:73-81 and _alg3d_bppt.py:931-944).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace odt {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+  R re, im;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> mk(R a, R b) {
+  cpx<R> c;
+  c.re = a;
+  c.im = b;
+  return c;
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re + b.re, a.im + b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re - b.re, a.im - b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator*(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator*(R s, cpx<R> a) {
+  return mk<R>(s * a.re, s * a.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> conj_if(cpx<R> a, bool c) {
+  return c ? mk<R>(a.re, -a.im) : a;
+}
+
+// ---------------------------------------------------------------------------
+// LDS transforms.  A workgroup transforms `nb` vectors of length n stored
+// contiguously in LDS buffer x (scratch y of the same size).  tw holds
+// exp(+2 pi i m / n), m = 0..n-1; dir < 0 uses the conjugate (forward).
+// Returns the buffer holding the result.  Both end with __syncthreads().
+// ---------------------------------------------------------------------------
+
+// Stockham autosort radix-2: pass s combines sub-transforms of length
+// Ls = 2^s; butterfly j (k = j mod Ls) reads x[j], x[j + n/2] and writes
+// y[2(j-k) + k] and y[2(j-k) + k + Ls] with twiddle exp(i*dir*pi*k/Ls).
+template <typename R>
+__device__ cpx<R> *lds_fft_pow2(cpx<R> *x, cpx<R> *y, int n, int logn, int nb,
+                                const cpx<R> *__restrict__ tw, int dir) {
+  const int half = n >> 1;
+  const int total = half * nb;
+  for (int s = 0; s < logn; ++s) {
+    const int ls = 1 << s;
+    const int tstride = n >> (s + 1);
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int b = idx >> (logn - 1);
+      const int j = idx & (half - 1);
+      const int k = j & (ls - 1);
+      const cpx<R> a = x[b * n + j];
+      const cpx<R> c = x[b * n + j + half];
+      const cpx<R> w = conj_if(tw[k * tstride], dir < 0);
+      const cpx<R> wc = w * c;
+      const int o = b * n + ((j - k) << 1) + k;
+      y[o] = a + wc;
+      y[o + ls] = a - wc;
+    }
+    __syncthreads();
+    cpx<R> *t = x;
+    x = y;
+    y = t;
+  }
+  return x;
+}
+
+// Direct DFT for lengths that are not powers of two (no-padding case).
+template <typename R>
+__device__ cpx<R> *lds_dft(cpx<R> *x, cpx<R> *y, int n, int nb,
+                           const cpx<R> *__restrict__ tw, int dir) {
+  for (int idx = threadIdx.x; idx < n * nb; idx += blockDim.x) {
+    const int b = idx / n;
+    const int m = idx - b * n;
+    cpx<R> acc = mk<R>(0, 0);
+    int p = 0;
+    for (int j = 0; j < n; ++j) {
+      acc = acc + x[b * n + j] * conj_if(tw[p], dir < 0);
+      p += m;
+      if (p >= n) p -= n;
+    }
+    y[idx] = acc;
+  }
+  __syncthreads();
+  return y;
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> *lds_transform(cpx<R> *x, cpx<R> *y, int n,
+                                                 int logn, int nb,
+                                                 const cpx<R> *tw, int dir) {
+  if (logn >= 0) return lds_fft_pow2<R>(x, y, n, logn, nb, tw, dir);
+  return lds_dft<R>(x, y, n, nb, tw, dir);
+}
+
+// ---------------------------------------------------------------------------
+// scipy.ndimage resampling rules (restated in oracle/bspline.py).
+// ---------------------------------------------------------------------------
+
+// Whole-sample-symmetric extension of a tap index onto [0, n-1].
+__device__ __forceinline__ int mirror_idx(int idx, int n) {
+  if (n <= 1) return 0;
+  const int s2 = 2 * n - 2;
+  if (idx < 0) {
+    idx = s2 * (-idx / s2) + idx;
+    idx = idx <= 1 - n ? idx + s2 : -idx;
+  } else if (idx >= n) {
+    idx -= s2 * (idx / s2);
+    if (idx >= n) idx = s2 - idx;
+  }
+  return idx;
+}
+
+// First tap and weights of the centred B-spline of degree ORDER at
+// coordinate c (float64).  Taps are start .. start+ORDER.
+template <int ORDER>
+__device__ __forceinline__ int spline_start(double c) {
+  return (ORDER & 1) ? (int)floor(c) - ORDER / 2
+                     : (int)floor(c + 0.5) - ORDER / 2;
+}
+
+template <int ORDER, typename W>
+__device__ __forceinline__ void spline_weights(double c, W *w) {
+  const double t = (ORDER & 1) ? c - floor(c) : c - floor(c + 0.5);
+  if (ORDER == 0) {
+    w[0] = W(1);
+  } else if (ORDER == 1) {
+    w[0] = W(1.0 - t);
+    w[1] = W(t);
+  } else if (ORDER == 2) {
+    // t in [-0.5, 0.5): taps at distance 1+t, t, 1-t
+    const double a = 0.5 - t;
+    w[0] = W(0.5 * a * a);
+    w[1] = W(0.75 - t * t);
+    w[2] = W(1.0 - 0.5 * a * a - (0.75 - t * t));
+  } else if (ORDER == 3) {
+    // t in [0, 1): taps at distance 1+t, t, 1-t, 2-t
+    const double z = 1.0 - t;
+    const double w1 = (t * t * (t - 2.0) * 3.0 + 4.0) / 6.0;
+    const double w2 = (z * z * (z - 2.0) * 3.0 + 4.0) / 6.0;
+    const double w0 = z * z * z / 6.0;
+    w[0] = W(w0);
+    w[1] = W(w1);
+    w[2] = W(w2);
+    w[3] = W(1.0 - w0 - w1 - w2);
+  } else if (ORDER == 4) {
+    // distances d = 2+t, 1+t, t, 1-t, 2-t with t in [-0.5, 0.5)
+    double s = 0.0;
+    for (int j = 0; j < 4; ++j) {
+      const double d = fabs(t + 2.0 - j);
+      double v;
+      if (d < 0.5) {
+        const double d2 = d * d;
+        v = d2 * (d2 * 0.25 - 0.625) + 115.0 / 192.0;
+      } else if (d < 1.5) {
+        v = (55.0 + d * (20.0 + d * (-120.0 + d * (80.0 - 16.0 * d)))) /
+            96.0;
+      } else {
+        const double e = 2.5 - d;
+        v = e * e * e * e / 24.0;
+      }
+      w[j] = W(v);
+      s += v;
+    }
+    w[4] = W(1.0 - s);
+  } else {  // ORDER == 5, t in [0, 1), distances 2+t .. 3-t
+    double s = 0.0;
+    for (int j = 0; j < 5; ++j) {
+      const double d = fabs(t + 2.0 - j);
+      double v;
+      if (d < 1.0) {
+        const double d2 = d * d;
+        v = d2 * (d2 * (0.25 - d / 12.0) - 0.5) + 0.55;
+      } else if (d < 2.0) {
+        v = d * (d * (d * (d * (d / 24.0 - 0.375) + 1.25) - 1.75) + 0.625) +
+            0.425;
+      } else {
+        const double e = 3.0 - d;
+        v = e * e * e * e * e / 120.0;
+      }
+      w[j] = W(v);
+      s += v;
+    }
+    w[5] = W(1.0 - s);
+  }
+}
+
+// Input coordinate of an affine map with scipy.ndimage's operation order:
+// ((0 + o0*m0) + o1*m1 [+ o2*m2]) + off, no fused multiply-add.
+__device__ __forceinline__ double affine2(double o0, double o1, double m0,
+                                          double m1, double off) {
+#pragma clang fp contract(off)
+  double t = o0 * m0;
+  t = t + o1 * m1;
+  return t + off;
+}
+__device__ __forceinline__ double affine3(double o0, double o1, double o2,
+                                          double m0, double m1, double m2,
+                                          double off) {
+#pragma clang fp contract(off)
+  double t = o0 * m0;
+  t = t + o1 * m1;
+  t = t + o2 * m2;
+  return t + off;
+}
+
+// Compensated (Kahan) accumulation.
+template <typename R>
+__device__ __forceinline__ void kahan_add(R &sum, R &comp, R v) {
+  const R y = v - comp;
+  const R t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+}
+
+}  // namespace odt

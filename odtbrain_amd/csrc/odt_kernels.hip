@@ -1,0 +1,742 @@
+// gfx950 kernels of the filtered-backpropagation path.
+//
+// Data flow for one chunk of projections (reference: the angle loop of
+// odtbrain/_alg3d_bpp.py:492-549, _alg3d_bppt.py:865-947, _alg2d_bpp.py:301-324):
+//
+//   k_fwd_rows   pad (np.pad edge / linear_ramp, weights folded in) and
+//                forward transform along x; keeps only the x-frequencies
+//                inside the low-pass support, stored column-major (F1)
+//   k_fwd_cols   forward transform along y, multiply by the prefactor table
+//                (ramp, low-pass, lD phase, 1/(lNx*lNy), and the fused
+//                B-spline prefilter factors); keeps the support box (S)
+//   k_col_ifft   per (x-frequency, z): propagator exp(i*km*(M-1)*z) (with
+//                the fused z-prefilter), inverse transform along y, crop (T)
+//   k_row_ifft   inverse transform along x, crop, fused x-prefilter; writes
+//                the B-spline coefficient slab of the projection
+//   k_rotate_acc / k_affine_acc
+//                resample every slab of the batch at the rotated position
+//                of each output voxel and sum over angles (compensated)
+//
+// k_iir is the explicit separable prefilter (used when the prefilter is not
+// folded into the transforms), k_fft_batch backs the standalone transform
+// entry point.
+#include "odt_device.h"
+#include "odt_launch.h"
+
+namespace odt {
+
+extern __shared__ __align__(16) unsigned char odt_smem[];
+
+// ---------------------------------------------------------------------------
+// twiddles: tw[m] = exp(+2 pi i m / n)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_twiddles(cpx<R> *tw, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double s, c;
+  sincospi(2.0 * (double)m / (double)n, &s, &c);
+  tw[m] = mk<R>((R)c, (R)s);
+}
+
+// ---------------------------------------------------------------------------
+// padding (numpy.pad semantics as called at _alg3d_bpp.py:496-506):
+// axis 0 (y) is padded first within the original columns, then axis 1 (x)
+// over all rows; edge mode replicates the nearest sample; linear_ramp goes
+// from `end` at the outer border to the edge value, excluding the edge:
+// value_i = end + i*(edge - end)/width, i = 0 at the outer border.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ cpx<R> ramp_val(cpx<R> edge, cpx<R> end, int i,
+                                           int width) {
+  const R f = (R)i / (R)width;
+  return mk<R>(end.re + f * (edge.re - end.re), end.im + f * (edge.im - end.im));
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> ramp_row(const cpx<R> *u, const Geom &g,
+                                           R w, cpx<R> end, int j, int col) {
+  const int Ny = g.Ny, Nx = g.Nx;
+  if (j < g.padyl) {
+    const cpx<R> e = w * u[col];
+    return ramp_val(e, end, j, g.padyl);
+  }
+  if (j >= g.padyl + Ny) {
+    const int padyr = g.lNy - Ny - g.padyl;
+    const int r = j - g.padyl - Ny;
+    const cpx<R> e = w * u[(Ny - 1) * Nx + col];
+    return ramp_val(e, end, padyr - 1 - r, padyr);
+  }
+  return w * u[(j - g.padyl) * Nx + col];
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> padded_value(const cpx<R> *u, const Geom &g,
+                                               R w, cpx<R> end, int j, int i) {
+  const int Ny = g.Ny, Nx = g.Nx;
+  if (g.pad_mode == 0) return w * u[j * Nx + i];
+  if (g.pad_mode == 1) {
+    int jj = j - g.padyl, ii = i - g.padxl;
+    jj = jj < 0 ? 0 : (jj >= Ny ? Ny - 1 : jj);
+    ii = ii < 0 ? 0 : (ii >= Nx ? Nx - 1 : ii);
+    return w * u[jj * Nx + ii];
+  }
+  if (i < g.padxl) {
+    const cpx<R> e = ramp_row(u, g, w, end, j, 0);
+    return ramp_val(e, end, i, g.padxl);
+  }
+  if (i >= g.padxl + Nx) {
+    const int padxr = g.lNx - Nx - g.padxl;
+    const int r = i - g.padxl - Nx;
+    const cpx<R> e = ramp_row(u, g, w, end, j, Nx - 1);
+    return ramp_val(e, end, padxr - 1 - r, padxr);
+  }
+  return ramp_row(u, g, w, end, j, i - g.padxl);
+}
+
+__device__ __forceinline__ int wrap(int j, int n) {
+  j %= n;
+  return j < 0 ? j + n : j;
+}
+
+// ---------------------------------------------------------------------------
+// forward transforms
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_fwd_rows(const cpx<R> *__restrict__ usin,
+               const double *__restrict__ weights, cpx<R> *__restrict__ F1,
+               const cpx<R> *__restrict__ twx, Geom g, double pad_re,
+               double pad_im, int a0, int nb) {
+  cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *tmp = buf + nb * g.lNx;
+  const int ai = blockIdx.y;
+  const int a = a0 + ai;
+  const int j0 = blockIdx.x * nb;
+  const cpx<R> *u = usin + (size_t)a * g.Ny * g.Nx;
+  const R w = (R)weights[a];
+  const cpx<R> end = mk<R>((R)pad_re, (R)pad_im);
+  for (int idx = threadIdx.x; idx < nb * g.lNx; idx += blockDim.x) {
+    const int r = idx / g.lNx, i = idx - r * g.lNx;
+    const int j = j0 + r;
+    buf[idx] = j < g.lNy ? padded_value(u, g, w, end, j, i) : mk<R>(0, 0);
+  }
+  __syncthreads();
+  const cpx<R> *res = lds_transform<R>(buf, tmp, g.lNx, g.logNx, nb, twx, -1);
+  for (int idx = threadIdx.x; idx < g.Lx * nb; idx += blockDim.x) {
+    const int c = idx / nb, r = idx - c * nb;
+    const int j = j0 + r;
+    if (j < g.lNy) {
+      const int bin = wrap(g.jx0 + c, g.lNx);
+      F1[((size_t)ai * g.Lx + c) * g.lNy + j] = res[r * g.lNx + bin];
+    }
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_fwd_cols(const cpx<R> *__restrict__ F1, cpx<R> *__restrict__ S,
+               const cpx<R> *__restrict__ twy, const cpx<R> *__restrict__ P,
+               Geom g, int nb) {
+  cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *tmp = buf + nb * g.lNy;
+  const int ai = blockIdx.y;
+  const int c0 = blockIdx.x * nb;
+  for (int idx = threadIdx.x; idx < nb * g.lNy; idx += blockDim.x) {
+    const int r = idx / g.lNy, j = idx - r * g.lNy;
+    const int c = c0 + r;
+    buf[idx] = c < g.Lx ? F1[((size_t)ai * g.Lx + c) * g.lNy + j] : mk<R>(0, 0);
+  }
+  __syncthreads();
+  const cpx<R> *res = lds_transform<R>(buf, tmp, g.lNy, g.logNy, nb, twy, -1);
+  for (int idx = threadIdx.x; idx < nb * g.Ly; idx += blockDim.x) {
+    const int r = idx / g.Ly, cy = idx - r * g.Ly;
+    const int c = c0 + r;
+    if (c < g.Lx) {
+      const int bin = wrap(g.jy0 + cy, g.lNy);
+      S[((size_t)ai * g.Lx + c) * g.Ly + cy] =
+          res[r * g.lNy + bin] * P[(size_t)c * g.Ly + cy];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// propagator exp(i*phi), phi = km*(M-1)*z computed in float64 and reduced
+// modulo 2*pi before the single-precision sincos
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ cpx<float> phasor(double phi, float) {
+  const double r = phi - 6.283185307179586 * rint(phi * 0.15915494309189535);
+  float s, c;
+  sincosf((float)r, &s, &c);
+  return mk<float>(c, s);
+}
+__device__ __forceinline__ cpx<double> phasor(double phi, double) {
+  double s, c;
+  sincos(phi, &s, &c);
+  return mk<double>(c, s);
+}
+
+// Column stage: for x-frequency column c and slices k0..k0+nb-1 build
+// S*g_k along the y-frequencies (zero outside the support), inverse
+// transform along y, crop rows padyl..padyl+Ny-1 (plus the fused
+// y-prefilter boundary terms for the tilted path), write T[a][k][c][y].
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_col_ifft(const cpx<R> *__restrict__ S, cpx<R> *__restrict__ T,
+               const cpx<R> *__restrict__ twy,
+               const double *__restrict__ prop_a,
+               const PropAux<R> *__restrict__ aux,
+               const R *__restrict__ q0z, const R *__restrict__ q0y, Geom g,
+               int nb) {
+  cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *tmp = buf + nb * g.lNy;
+  __shared__ cpx<R> corr[2 * 16];
+  const int c = blockIdx.x;
+  const int k0 = blockIdx.y * nb;
+  const int ai = blockIdx.z;
+  const int ln = g.ln;
+  for (int idx = threadIdx.x; idx < nb * g.lNy; idx += blockDim.x)
+    buf[idx] = mk<R>(0, 0);
+  __syncthreads();
+  const cpx<R> *Sc = S + ((size_t)ai * g.Lx + c) * g.Ly;
+  for (int idx = threadIdx.x; idx < nb * g.Ly; idx += blockDim.x) {
+    const int r = idx / g.Ly, cy = idx - r * g.Ly;
+    const int k = k0 + r;
+    if (k >= ln) continue;
+    const size_t t = (size_t)c * g.Ly + cy;
+    const cpx<R> e = phasor(prop_a[t] * (g.z0 + (double)k), R(0));
+    cpx<R> gk = e;
+    if (g.fuse_z) {
+      const PropAux<R> x = aux[t];
+      gk = x.ib * e + q0z[k] * x.r0 + q0z[ln - 1 - k] * x.r1;
+    }
+    buf[r * g.lNy + wrap(g.jy0 + cy, g.lNy)] = Sc[cy] * gk;
+  }
+  __syncthreads();
+  const cpx<R> *res = lds_transform<R>(buf, tmp, g.lNy, g.logNy, nb, twy, +1);
+  const int ny = g.Ny;
+  if (g.fuse_y) {
+    if (threadIdx.x < nb) {
+      const cpx<R> *v = res + threadIdx.x * g.lNy;
+      const R b1 = (R)g.b1;
+      corr[2 * threadIdx.x] =
+          b1 * (v[wrap(g.padyl - 1, g.lNy)] - v[wrap(g.padyl + 1, g.lNy)]);
+      corr[2 * threadIdx.x + 1] = b1 * (v[wrap(g.padyl + ny, g.lNy)] -
+                                        v[wrap(g.padyl + ny - 2, g.lNy)]);
+    }
+    __syncthreads();
+  }
+  for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) {
+    const int r = idx / ny, y = idx - r * ny;
+    const int k = k0 + r;
+    if (k >= ln) continue;
+    cpx<R> v = res[r * g.lNy + wrap(g.padyl + y, g.lNy)];
+    if (g.fuse_y)
+      v = v + q0y[y] * corr[2 * r] + q0y[ny - 1 - y] * corr[2 * r + 1];
+    T[(((size_t)ai * ln + k) * g.Lx + c) * ny + y] = v;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void store_val(R *p, cpx<R> v) { *p = v.re; }
+template <typename R>
+__device__ __forceinline__ void store_val(cpx<R> *p, cpx<R> v) { *p = v; }
+
+// Row stage: rows y0..y0+nb-1 of slice k: inverse transform along x of the
+// support columns of T, crop columns padxl..padxl+Nx-1 with the fused
+// x-prefilter boundary terms, write the coefficient slab [s][y][k][x].
+template <typename R, typename V>
+__global__ void __launch_bounds__(256)
+    k_row_ifft(const cpx<R> *__restrict__ T, V *__restrict__ slab,
+               const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
+               Geom g, int nb, int slab0) {
+  cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *tmp = buf + nb * g.lNx;
+  __shared__ cpx<R> corr[2 * 16];
+  const int y0 = blockIdx.x * nb;
+  const int k = blockIdx.y;
+  const int ai = blockIdx.z;
+  const int ny = g.Ny, nx = g.Nx, ln = g.ln;
+  for (int idx = threadIdx.x; idx < nb * g.lNx; idx += blockDim.x)
+    buf[idx] = mk<R>(0, 0);
+  __syncthreads();
+  const cpx<R> *Tk = T + ((size_t)ai * ln + k) * g.Lx * ny;
+  for (int idx = threadIdx.x; idx < g.Lx * nb; idx += blockDim.x) {
+    const int c = idx / nb, r = idx - c * nb;
+    const int y = y0 + r;
+    if (y < ny) buf[r * g.lNx + wrap(g.jx0 + c, g.lNx)] = Tk[(size_t)c * ny + y];
+  }
+  __syncthreads();
+  const cpx<R> *res = lds_transform<R>(buf, tmp, g.lNx, g.logNx, nb, twx, +1);
+  if (g.fuse_x) {
+    if (threadIdx.x < nb) {
+      const cpx<R> *v = res + threadIdx.x * g.lNx;
+      const R b1 = (R)g.b1;
+      corr[2 * threadIdx.x] =
+          b1 * (v[wrap(g.padxl - 1, g.lNx)] - v[wrap(g.padxl + 1, g.lNx)]);
+      corr[2 * threadIdx.x + 1] = b1 * (v[wrap(g.padxl + nx, g.lNx)] -
+                                        v[wrap(g.padxl + nx - 2, g.lNx)]);
+    }
+    __syncthreads();
+  }
+  const int s = slab0 + ai;
+  for (int idx = threadIdx.x; idx < nb * nx; idx += blockDim.x) {
+    const int r = idx / nx, x = idx - r * nx;
+    const int y = y0 + r;
+    if (y >= ny) continue;
+    cpx<R> v = res[r * g.lNx + wrap(g.padxl + x, g.lNx)];
+    if (g.fuse_x)
+      v = v + q0x[x] * corr[2 * r] + q0x[nx - 1 - x] * corr[2 * r + 1];
+    store_val(slab + (((size_t)s * ny + y) * ln + k) * nx + x, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rotate-accumulate about the y axis (scipy.ndimage.rotate(axes=(0,2)) of
+// every y plane, _alg3d_bpp.py:68-81, summed over the angles of the batch)
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Acc {
+  R s, c;
+};
+template <typename R>
+__device__ __forceinline__ void acc_zero(R &v) { v = R(0); }
+template <typename R>
+__device__ __forceinline__ void acc_zero(cpx<R> &v) { v = mk<R>(0, 0); }
+template <typename R>
+__device__ __forceinline__ void acc_add(R &s, R &c, R v) { kahan_add(s, c, v); }
+template <typename R>
+__device__ __forceinline__ void acc_add(cpx<R> &s, cpx<R> &c, cpx<R> v) {
+  kahan_add(s.re, c.re, v.re);
+  kahan_add(s.im, c.im, v.im);
+}
+template <typename R>
+__device__ __forceinline__ R fma_w(R w, R v, R acc) { return acc + w * v; }
+template <typename R>
+__device__ __forceinline__ cpx<R> fma_w(R w, cpx<R> v, cpx<R> acc) {
+  return mk<R>(acc.re + w * v.re, acc.im + w * v.im);
+}
+
+template <typename R, typename V, int ORDER, int YC>
+__global__ void __launch_bounds__(256)
+    k_rotate_acc(const V *__restrict__ coef, int nb,
+                 const double *__restrict__ params, V *__restrict__ out,
+                 V *__restrict__ comp, int ny, int nz, int nx, int first,
+                 int last) {
+  const int x = blockIdx.x * 16 + threadIdx.x;
+  const int z = blockIdx.y * 16 + threadIdx.y;
+  const int y0 = blockIdx.z * YC;
+  if (x >= nx || z >= nz) return;
+  V sum[YC], cmp[YC];
+#pragma unroll
+  for (int q = 0; q < YC; ++q) {
+    const int y = y0 + q;
+    if (!first && y < ny) {
+      const size_t o = ((size_t)z * ny + y) * nx + x;
+      sum[q] = out[o];
+      if (comp)
+        cmp[q] = comp[o];
+      else
+        acc_zero(cmp[q]);
+    } else {
+      acc_zero(sum[q]);
+      acc_zero(cmp[q]);
+    }
+  }
+  const size_t plane = (size_t)nz * nx;
+  const double oz = (double)z, ox = (double)x;
+  for (int a = 0; a < nb; ++a) {
+    const double *p = params + 6 * a;
+    const double cz = affine2(oz, ox, p[0], p[1], p[4]);
+    const double cx = affine2(oz, ox, p[2], p[3], p[5]);
+    if (cz < 0.0 || cz > (double)(nz - 1) || cx < 0.0 || cx > (double)(nx - 1))
+      continue;
+    R wz[ORDER + 1], wx[ORDER + 1];
+    int iz[ORDER + 1], ix[ORDER + 1];
+    spline_weights<ORDER, R>(cz, wz);
+    spline_weights<ORDER, R>(cx, wx);
+    const int sz = spline_start<ORDER>(cz), sx = spline_start<ORDER>(cx);
+#pragma unroll
+    for (int j = 0; j <= ORDER; ++j) {
+      iz[j] = mirror_idx(sz + j, nz) * nx;
+      ix[j] = mirror_idx(sx + j, nx);
+    }
+    const V *base = coef + (size_t)a * ny * plane;
+#pragma unroll
+    for (int q = 0; q < YC; ++q) {
+      const int y = y0 + q;
+      if (y >= ny) break;
+      const V *pl = base + (size_t)y * plane;
+      V v;
+      acc_zero(v);
+#pragma unroll
+      for (int i = 0; i <= ORDER; ++i) {
+        V t;
+        acc_zero(t);
+#pragma unroll
+        for (int j = 0; j <= ORDER; ++j) t = fma_w(wx[j], pl[iz[i] + ix[j]], t);
+        v = fma_w(wz[i], t, v);
+      }
+      acc_add(sum[q], cmp[q], v);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < YC; ++q) {
+    const int y = y0 + q;
+    if (y >= ny) break;
+    const size_t o = ((size_t)z * ny + y) * nx + x;
+    out[o] = sum[q];
+    if (!last && comp) comp[o] = cmp[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// affine gather-accumulate for the tilted axis (_alg3d_bppt.py:904-944):
+// output voxel (z,y,x) is sample o = (x, Ny-1-y, z) of the (x, -y, z) frame;
+// input index c = Dinv @ o + offset in that frame; coefficient (i0,i1,i2)
+// of the frame lives at slab[y = Ny-1-i1][z = i2][x = i0].
+// ---------------------------------------------------------------------------
+template <typename R, typename V, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_affine_acc(const V *__restrict__ coef, int nb,
+                 const double *__restrict__ params, V *__restrict__ out,
+                 V *__restrict__ comp, int ny, int nz, int nx, int first,
+                 int last) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  const size_t o = ((size_t)z * ny + y) * nx + x;
+  V sum, cmp;
+  if (!first) {
+    sum = out[o];
+    if (comp)
+      cmp = comp[o];
+    else
+      acc_zero(cmp);
+  } else {
+    acc_zero(sum);
+    acc_zero(cmp);
+  }
+  const double o0 = (double)x, o1 = (double)(ny - 1 - y), o2 = (double)z;
+  const size_t vol = (size_t)ny * nz * nx;
+  for (int a = 0; a < nb; ++a) {
+    const double *p = params + 12 * a;
+    const double c0 = affine3(o0, o1, o2, p[0], p[1], p[2], p[9]);
+    const double c1 = affine3(o0, o1, o2, p[3], p[4], p[5], p[10]);
+    const double c2 = affine3(o0, o1, o2, p[6], p[7], p[8], p[11]);
+    if (c0 < 0.0 || c0 > (double)(nx - 1) || c1 < 0.0 ||
+        c1 > (double)(ny - 1) || c2 < 0.0 || c2 > (double)(nz - 1))
+      continue;
+    R w0[ORDER + 1], w1[ORDER + 1], w2[ORDER + 1];
+    int i0[ORDER + 1];
+    size_t i1[ORDER + 1], i2[ORDER + 1];
+    spline_weights<ORDER, R>(c0, w0);
+    spline_weights<ORDER, R>(c1, w1);
+    spline_weights<ORDER, R>(c2, w2);
+    const int s0 = spline_start<ORDER>(c0), s1 = spline_start<ORDER>(c1),
+              s2 = spline_start<ORDER>(c2);
+#pragma unroll
+    for (int j = 0; j <= ORDER; ++j) {
+      i0[j] = mirror_idx(s0 + j, nx);
+      i1[j] = (size_t)(ny - 1 - mirror_idx(s1 + j, ny)) * nz * nx;
+      i2[j] = (size_t)mirror_idx(s2 + j, nz) * nx;
+    }
+    const V *base = coef + (size_t)a * vol;
+    V v;
+    acc_zero(v);
+#pragma unroll
+    for (int q = 0; q <= ORDER; ++q) {
+      V tq;
+      acc_zero(tq);
+#pragma unroll
+      for (int r = 0; r <= ORDER; ++r) {
+        const V *row = base + i1[q] + i2[r];
+        V t;
+        acc_zero(t);
+#pragma unroll
+        for (int pp = 0; pp <= ORDER; ++pp) t = fma_w(w0[pp], row[i0[pp]], t);
+        tq = fma_w(w2[r], t, tq);
+      }
+      v = fma_w(w1[q], tq, v);
+    }
+    acc_add(sum, cmp, v);
+  }
+  out[o] = sum;
+  if (!last && comp) comp[o] = cmp;
+}
+
+// ---------------------------------------------------------------------------
+// separable mirror-boundary B-spline prefilter, one thread per line
+// (Unser/Thevenaz recursive filter; exact mirror initialisation), computed
+// in float64.  Line l starts at (l / n0) * s1 + (l % n0) * s0, elements es.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_iir(R *data, long long nlines, int n, long long n0,
+                      long long s0, long long s1, long long es, int npoles,
+                      double z1, double z2, double gain) {
+  const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nlines || n < 2) return;
+  R *p = data + (l / n0) * s1 + (l % n0) * s0;
+  for (int i = 0; i < n; ++i) p[i * es] = (R)((double)p[i * es] * gain);
+  for (int pi = 0; pi < npoles; ++pi) {
+    const double z = pi == 0 ? z1 : z2;
+    const double zn1 = pow(z, (double)(n - 1));
+    double c0 = (double)p[0] + zn1 * (double)p[(n - 1) * es];
+    double zi = z;
+    for (int i = 1; i < n - 1; ++i) {
+      c0 += zi * ((double)p[i * es] + zn1 * (double)p[(n - 1 - i) * es]);
+      zi *= z;
+    }
+    double prev = c0 / (1.0 - zn1 * zn1);
+    p[0] = (R)prev;
+    for (int i = 1; i < n; ++i) {
+      prev = (double)p[i * es] + z * prev;
+      p[i * es] = (R)prev;
+    }
+    double next = (z * (double)p[(n - 2) * es] + prev) * z / (z * z - 1.0);
+    p[(n - 1) * es] = (R)next;
+    for (int i = n - 2; i >= 0; --i) {
+      next = z * (next - (double)p[i * es]);
+      p[i * es] = (R)next;
+    }
+  }
+}
+
+// batched 1D transform of contiguous vectors
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch, int n,
+                int logn, int dir, const cpx<R> *tw, int nb) {
+  cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *tmp = buf + nb * n;
+  const long long b0 = (long long)blockIdx.x * nb;
+  for (int idx = threadIdx.x; idx < nb * n; idx += blockDim.x) {
+    const long long b = b0 + idx / n;
+    buf[idx] = b < batch ? in[b * n + idx % n] : mk<R>(0, 0);
+  }
+  __syncthreads();
+  const cpx<R> *res = lds_transform<R>(buf, tmp, n, logn, nb, tw, dir);
+  for (int idx = threadIdx.x; idx < nb * n; idx += blockDim.x) {
+    const long long b = b0 + idx / n;
+    if (b < batch) out[b * n + idx % n] = res[idx];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static size_t lds_bytes(int n, int nb, size_t esz) {
+  return (size_t)2 * n * nb * esz;
+}
+
+static hipError_t set_lds(const void *fn, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+template <typename R>
+hipError_t launch_twiddles(cpx<R> *tw, int n, hipStream_t st) {
+  k_twiddles<R><<<(n + 255) / 256, 256, 0, st>>>(tw, n);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fwd_rows(const cpx<R> *usin, const double *w, cpx<R> *F1,
+                           const cpx<R> *twx, const Geom &g, double pr,
+                           double pi, int a0, int na, int nb,
+                           hipStream_t st) {
+  const size_t sh = lds_bytes(g.lNx, nb, sizeof(cpx<R>));
+  hipError_t e = set_lds((const void *)k_fwd_rows<R>, sh);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.lNy + nb - 1) / nb, na);
+  k_fwd_rows<R><<<grid, 256, sh, st>>>(usin, w, F1, twx, g, pr, pi, a0, nb);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fwd_cols(const cpx<R> *F1, cpx<R> *S, const cpx<R> *twy,
+                           const cpx<R> *P, const Geom &g, int na, int nb,
+                           hipStream_t st) {
+  const size_t sh = lds_bytes(g.lNy, nb, sizeof(cpx<R>));
+  hipError_t e = set_lds((const void *)k_fwd_cols<R>, sh);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.Lx + nb - 1) / nb, na);
+  k_fwd_cols<R><<<grid, 256, sh, st>>>(F1, S, twy, P, g, nb);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
+                           const double *pa, const PropAux<R> *aux,
+                           const R *q0z, const R *q0y, const Geom &g, int na,
+                           int nb, hipStream_t st) {
+  const size_t sh = lds_bytes(g.lNy, nb, sizeof(cpx<R>));
+  hipError_t e = set_lds((const void *)k_col_ifft<R>, sh);
+  if (e != hipSuccess) return e;
+  dim3 grid(g.Lx, (g.ln + nb - 1) / nb, na);
+  k_col_ifft<R><<<grid, 256, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g, nb);
+  return hipGetLastError();
+}
+
+template <typename R, typename V>
+hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
+                           const R *q0x, const Geom &g, int na, int nb,
+                           int slab0, hipStream_t st) {
+  const size_t sh = lds_bytes(g.lNx, nb, sizeof(cpx<R>));
+  hipError_t e = set_lds((const void *)k_row_ifft<R, V>, sh);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.Ny + nb - 1) / nb, g.ln, na);
+  k_row_ifft<R, V><<<grid, 256, sh, st>>>(T, slab, twx, q0x, g, nb, slab0);
+  return hipGetLastError();
+}
+
+template <typename R, typename V, int ORDER>
+static hipError_t rotate_dispatch_yc(const V *coef, int nb, const double *params,
+                                     V *out, V *comp, int ny, int nz, int nx,
+                                     int first, int last, hipStream_t st) {
+  dim3 block(16, 16);
+  if (ny >= 8) {
+    dim3 grid((nx + 15) / 16, (nz + 15) / 16, (ny + 7) / 8);
+    k_rotate_acc<R, V, ORDER, 8><<<grid, block, 0, st>>>(
+        coef, nb, params, out, comp, ny, nz, nx, first, last);
+  } else {
+    dim3 grid((nx + 15) / 16, (nz + 15) / 16, ny);
+    k_rotate_acc<R, V, ORDER, 1><<<grid, block, 0, st>>>(
+        coef, nb, params, out, comp, ny, nz, nx, first, last);
+  }
+  return hipGetLastError();
+}
+
+template <typename R, typename V>
+hipError_t launch_rotate_acc(int order, const V *coef, int nb,
+                             const double *params, V *out, V *comp, int ny,
+                             int nz, int nx, int first, int last,
+                             hipStream_t st) {
+  switch (order) {
+    case 0: return rotate_dispatch_yc<R, V, 0>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 1: return rotate_dispatch_yc<R, V, 1>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 2: return rotate_dispatch_yc<R, V, 2>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 3: return rotate_dispatch_yc<R, V, 3>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 4: return rotate_dispatch_yc<R, V, 4>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 5: return rotate_dispatch_yc<R, V, 5>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R, typename V, int ORDER>
+static hipError_t affine_dispatch(const V *coef, int nb, const double *params,
+                                  V *out, V *comp, int ny, int nz, int nx,
+                                  int first, int last, hipStream_t st) {
+  dim3 block(64, 4);
+  dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+  k_affine_acc<R, V, ORDER><<<grid, block, 0, st>>>(coef, nb, params, out, comp,
+                                                    ny, nz, nx, first, last);
+  return hipGetLastError();
+}
+
+template <typename R, typename V>
+hipError_t launch_affine_acc(int order, const V *coef, int nb,
+                             const double *params, V *out, V *comp, int ny,
+                             int nz, int nx, int first, int last,
+                             hipStream_t st) {
+  switch (order) {
+    case 0: return affine_dispatch<R, V, 0>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 1: return affine_dispatch<R, V, 1>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 2: return affine_dispatch<R, V, 2>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 3: return affine_dispatch<R, V, 3>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 4: return affine_dispatch<R, V, 4>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 5: return affine_dispatch<R, V, 5>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_iir(R *data, long long nlines, int n, long long n0,
+                      long long s0, long long s1, long long es, int order,
+                      hipStream_t st) {
+  double z1 = 0, z2 = 0;
+  int np = 1;
+  switch (order) {
+    case 2: z1 = sqrt(8.0) - 3.0; break;
+    case 3: z1 = sqrt(3.0) - 2.0; break;
+    case 4:
+      np = 2;
+      z1 = sqrt(664.0 - sqrt(438976.0)) + sqrt(304.0) - 19.0;
+      z2 = sqrt(664.0 + sqrt(438976.0)) - sqrt(304.0) - 19.0;
+      break;
+    case 5:
+      np = 2;
+      z1 = sqrt(67.5 - sqrt(4436.25)) + sqrt(26.25) - 6.5;
+      z2 = sqrt(67.5 + sqrt(4436.25)) - sqrt(26.25) - 6.5;
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  double gain = (1.0 - z1) * (1.0 - 1.0 / z1);
+  if (np == 2) gain *= (1.0 - z2) * (1.0 - 1.0 / z2);
+  if (nlines <= 0) return hipSuccess;
+  const long long blocks = (nlines + 255) / 256;
+  k_iir<R><<<(unsigned)blocks, 256, 0, st>>>(data, nlines, n, n0, s0, s1, es,
+                                            np, z1, z2, gain);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
+                            int n, int logn, int dir, const cpx<R> *tw, int nb,
+                            hipStream_t st) {
+  const size_t sh = lds_bytes(n, nb, sizeof(cpx<R>));
+  hipError_t e = set_lds((const void *)k_fft_batch<R>, sh);
+  if (e != hipSuccess) return e;
+  const long long blocks = (batch + nb - 1) / nb;
+  k_fft_batch<R><<<(unsigned)blocks, 256, sh, st>>>(in, out, batch, n, logn,
+                                                   dir, tw, nb);
+  return hipGetLastError();
+}
+
+// explicit instantiations
+#define ODT_INST(R)                                                            \
+  template hipError_t launch_twiddles<R>(cpx<R> *, int, hipStream_t);          \
+  template hipError_t launch_fwd_rows<R>(const cpx<R> *, const double *,       \
+                                         cpx<R> *, const cpx<R> *,             \
+                                         const Geom &, double, double, int,    \
+                                         int, int, hipStream_t);               \
+  template hipError_t launch_fwd_cols<R>(const cpx<R> *, cpx<R> *,             \
+                                         const cpx<R> *, const cpx<R> *,       \
+                                         const Geom &, int, int, hipStream_t); \
+  template hipError_t launch_col_ifft<R>(                                      \
+      const cpx<R> *, cpx<R> *, const cpx<R> *, const double *,                \
+      const PropAux<R> *, const R *, const R *, const Geom &, int, int,        \
+      hipStream_t);                                                            \
+  template hipError_t launch_row_ifft<R, R>(const cpx<R> *, R *,               \
+                                            const cpx<R> *, const R *,         \
+                                            const Geom &, int, int, int,       \
+                                            hipStream_t);                      \
+  template hipError_t launch_row_ifft<R, cpx<R>>(                              \
+      const cpx<R> *, cpx<R> *, const cpx<R> *, const R *, const Geom &, int,  \
+      int, int, hipStream_t);                                                  \
+  template hipError_t launch_rotate_acc<R, R>(int, const R *, int,             \
+                                              const double *, R *, R *, int,   \
+                                              int, int, int, int,              \
+                                              hipStream_t);                    \
+  template hipError_t launch_rotate_acc<R, cpx<R>>(                            \
+      int, const cpx<R> *, int, const double *, cpx<R> *, cpx<R> *, int, int,  \
+      int, int, int, hipStream_t);                                             \
+  template hipError_t launch_affine_acc<R, R>(int, const R *, int,             \
+                                              const double *, R *, R *, int,   \
+                                              int, int, int, int,              \
+                                              hipStream_t);                    \
+  template hipError_t launch_affine_acc<R, cpx<R>>(                            \
+      int, const cpx<R> *, int, const double *, cpx<R> *, cpx<R> *, int, int,  \
+      int, int, int, hipStream_t);                                             \
+  template hipError_t launch_iir<R>(R *, long long, int, long long, long long, \
+                                    long long, long long, int, hipStream_t);   \
+  template hipError_t launch_fft_batch<R>(const cpx<R> *, cpx<R> *, long long, \
+                                          int, int, int, const cpx<R> *, int,  \
+                                          hipStream_t);
+
+ODT_INST(float)
+ODT_INST(double)
+
+}  // namespace odt

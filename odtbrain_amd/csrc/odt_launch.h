@@ -1,0 +1,71 @@
+// Internal interface between the plan (odt_plan.cpp) and the kernels
+// (odt_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "odt_device.h"
+
+namespace odt {
+
+// Sizes shared by the transform kernels of one plan.
+struct Geom {
+  int A, Ny, Nx, ln;     // projections, detector rows/cols, z slices (= Nx)
+  int lNy, lNx;          // padded transform sizes
+  int logNy, logNx;      // log2 of the sizes, -1 if not a power of two
+  int padyl, padxl;      // leading pad widths
+  int pad_mode;          // 0 none, 1 edge, 2 linear ramp
+  int Lx, Ly;            // support box (x- and y-frequencies kept)
+  int jx0, jy0;          // signed frequency index of support column/row 0
+  int fuse_x, fuse_y, fuse_z;  // prefilter folded into the transforms
+  double b1;             // B-spline sample at +-1 (1/8 order 2, 1/6 order 3)
+  double z0;             // z of slice 0
+};
+
+// Per support point: fused z-prefilter terms (see k_col_ifft).
+template <typename R>
+struct PropAux {
+  cpx<R> r0, r1;
+  R ib;
+  R pad;
+};
+
+template <typename R>
+hipError_t launch_twiddles(cpx<R> *tw, int n, hipStream_t st);
+template <typename R>
+hipError_t launch_fwd_rows(const cpx<R> *usin, const double *w, cpx<R> *F1,
+                           const cpx<R> *twx, const Geom &g, double pr,
+                           double pi, int a0, int na, int nb, hipStream_t st);
+template <typename R>
+hipError_t launch_fwd_cols(const cpx<R> *F1, cpx<R> *S, const cpx<R> *twy,
+                           const cpx<R> *P, const Geom &g, int na, int nb,
+                           hipStream_t st);
+template <typename R>
+hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
+                           const double *pa, const PropAux<R> *aux,
+                           const R *q0z, const R *q0y, const Geom &g, int na,
+                           int nb, hipStream_t st);
+template <typename R, typename V>
+hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
+                           const R *q0x, const Geom &g, int na, int nb,
+                           int slab0, hipStream_t st);
+template <typename R, typename V>
+hipError_t launch_rotate_acc(int order, const V *coef, int nb,
+                             const double *params, V *out, V *comp, int ny,
+                             int nz, int nx, int first, int last,
+                             hipStream_t st);
+template <typename R, typename V>
+hipError_t launch_affine_acc(int order, const V *coef, int nb,
+                             const double *params, V *out, V *comp, int ny,
+                             int nz, int nx, int first, int last,
+                             hipStream_t st);
+template <typename R>
+hipError_t launch_iir(R *data, long long nlines, int n, long long n0,
+                      long long s0, long long s1, long long es, int order,
+                      hipStream_t st);
+template <typename R>
+hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
+                            int n, int logn, int dir, const cpx<R> *tw, int nb,
+                            hipStream_t st);
+
+}  // namespace odt

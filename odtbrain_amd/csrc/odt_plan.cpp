@@ -1,0 +1,578 @@
+// Plan construction and the C ABI of libodtbrain_hip.so (include/odtbrain_hip.h).
+//
+// A plan holds the float64-derived tables of one reconstruction call of the
+// reference (odtbrain/_alg3d_bpp.py:371-441, _alg3d_bppt.py:736-822,
+// _alg2d_bpp.py:239-287): twiddles, the low-pass support box of the
+// spectrum, the prefactor table, the propagator phase table km*(M-1), the
+// fused B-spline prefilter terms, and the per-angle weights and geometry.
+// Execution streams the projections through the transform kernels in
+// chunks and accumulates batches of coefficient slabs into the output.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/odtbrain_hip.h"
+#include "odt_launch.h"
+
+using odt::cpx;
+using odt::Geom;
+using odt::PropAux;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return ODT_ERR_HIP;
+}
+
+#define HIPCHK(expr, where)                   \
+  do {                                        \
+    hipError_t e_ = (expr);                   \
+    if (e_ != hipSuccess) return hip_fail(e_, where); \
+  } while (0)
+
+int ilog2_exact(int n) {
+  if (n <= 0 || (n & (n - 1))) return -1;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+// B-spline samples at 0 and +-1 (beta(theta) = b0 + 2 b1 cos theta).
+void bspline_b01(int order, double *b0, double *b1) {
+  if (order == 2) {
+    *b0 = 0.75;
+    *b1 = 0.125;
+  } else {
+    *b0 = 2.0 / 3.0;
+    *b1 = 1.0 / 6.0;
+  }
+}
+
+// First column of the inverse mirror-boundary B-spline sampling matrix:
+// the recursive prefilter (exact mirror initialisation) applied to delta_0.
+std::vector<double> prefilter_delta0(int order, int n) {
+  std::vector<double> c(n, 0.0);
+  c[0] = 1.0;
+  if (n < 2) return c;
+  const double z = order == 2 ? std::sqrt(8.0) - 3.0 : std::sqrt(3.0) - 2.0;
+  const double gain = (1.0 - z) * (1.0 - 1.0 / z);
+  for (auto &v : c) v *= gain;
+  const double zn1 = std::pow(z, n - 1);
+  double c0 = c[0] + zn1 * c[n - 1];
+  double zi = z;
+  for (int i = 1; i < n - 1; ++i) {
+    c0 += zi * (c[i] + zn1 * c[n - 1 - i]);
+    zi *= z;
+  }
+  c[0] = c0 / (1.0 - zn1 * zn1);
+  for (int i = 1; i < n; ++i) c[i] += z * c[i - 1];
+  c[n - 1] = (z * c[n - 2] + c[n - 1]) * z / (z * z - 1.0);
+  for (int i = n - 2; i >= 0; --i) c[i] = z * (c[i + 1] - c[i]);
+  return c;
+}
+
+// Frequency support along one axis: signed indices j (numpy.fft.fftfreq
+// order, j in [-(n/2), (n-1)/2]) with (2*pi*j/n)^2 < km^2.
+void support(int n, double km, int *j0, int *len) {
+  const int jmin = -(n / 2), jmax = (n - 1) / 2;
+  const double val = 1.0 / (double)n;
+  const double two_pi = 2.0 * M_PI;
+  int J = -1;
+  for (int j = 0; j <= jmax || j <= -jmin; ++j) {
+    const double k = two_pi * ((double)j * val);
+    if (k * k < km * km)
+      J = j;
+    else
+      break;
+  }
+  if (J < 0) {
+    *j0 = 0;
+    *len = 1;
+    return;
+  }
+  const int lo = std::max(jmin, -J), hi = std::min(jmax, J);
+  *j0 = lo;
+  *len = hi - lo + 1;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct odt_plan {
+  int kind, precision, order, onlyreal, flags, pstride, A;
+  double padval_re, padval_im;
+  Geom g;
+  bool explicit_iir;
+  int NB, NC;                 // slabs per accumulate batch, angles per chunk
+  int nb_fr, nb_fc, nb_col, nb_row;  // transforms per block
+  size_t esz_c, esz_v;        // bytes of complex element / slab element
+  size_t off_F1, off_S, off_T, off_slab, off_comp, ws_bytes;
+  // device tables
+  void *twx = nullptr, *twy = nullptr, *P = nullptr, *aux = nullptr;
+  void *q0x = nullptr, *q0y = nullptr, *q0z = nullptr;
+  double *prop_a = nullptr, *weights = nullptr, *params = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_rot;  // start/stop pairs
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+  int n_rot = 0;
+};
+
+namespace {
+
+template <typename R>
+int upload_tables(odt_plan *p, const odt_desc *d) {
+  const Geom &g = p->g;
+  const int Lx = g.Lx, Ly = g.Ly;
+  const double km = d->km, lD = d->lD;
+  const double two_pi = 2.0 * M_PI;
+  const double dphi0 = two_pi / (double)d->A;
+  double b0 = 0, b1 = 0;
+  const bool fused = !p->explicit_iir && p->order >= 2;
+  if (fused) bspline_b01(p->order, &b0, &b1);
+  p->g.b1 = b1;
+  const std::complex<double> pre0 =
+      std::complex<double>(0.0, -1.0) * km / two_pi * dphi0;
+  const double norm = 1.0 / ((double)g.lNx * (double)g.lNy);
+  std::vector<cpx<R>> P((size_t)Lx * Ly);
+  std::vector<double> pa((size_t)Lx * Ly);
+  std::vector<PropAux<R>> aux(g.fuse_z ? (size_t)Lx * Ly : 1);
+  const double valx = 1.0 / (double)g.lNx, valy = 1.0 / (double)g.lNy;
+  for (int c = 0; c < Lx; ++c) {
+    const int jx = g.jx0 + c;
+    const double kx = two_pi * ((double)jx * valx);
+    const double bx = g.fuse_x ? b0 + 2.0 * b1 * std::cos(two_pi * jx * valx) : 1.0;
+    for (int cy = 0; cy < Ly; ++cy) {
+      const int jy = g.jy0 + cy;
+      const double ky = two_pi * ((double)jy * valy);
+      const double klp = (kx * kx + ky * ky < km * km) ? 1.0 : 0.0;
+      const double M = 1.0 / km * std::sqrt((km * km - kx * kx - ky * ky) * klp);
+      const double ramp = std::fabs(kx * d->ramp_v + ky * d->ramp_u) * klp;
+      std::complex<double> pf = pre0 * ramp;
+      pf *= std::exp(std::complex<double>(0.0, -km * (M - 1.0) * lD));
+      pf *= norm;
+      if (g.fuse_x) pf /= bx;
+      if (g.fuse_y) pf /= b0 + 2.0 * b1 * std::cos(two_pi * jy * valy);
+      const size_t t = (size_t)c * Ly + cy;
+      P[t] = odt::mk<R>((R)pf.real(), (R)pf.imag());
+      const double a = km * (M - 1.0);
+      pa[t] = a;
+      if (g.fuse_z) {
+        const double beta = b0 + 2.0 * b1 * std::cos(a);
+        auto h = [&](double k) {
+          return std::exp(std::complex<double>(0.0, a * (g.z0 + k))) / beta;
+        };
+        const std::complex<double> r0 = b1 * (h(-1.0) - h(1.0));
+        const std::complex<double> r1 = b1 * (h((double)g.ln) - h((double)g.ln - 2.0));
+        aux[t].r0 = odt::mk<R>((R)r0.real(), (R)r0.imag());
+        aux[t].r1 = odt::mk<R>((R)r1.real(), (R)r1.imag());
+        aux[t].ib = (R)(1.0 / beta);
+        aux[t].pad = 0;
+      }
+    }
+  }
+  auto q0vec = [&](int n, bool on) {
+    std::vector<R> q(std::max(n, 1), (R)0);
+    if (on) {
+      auto qd = prefilter_delta0(p->order, n);
+      for (int i = 0; i < n; ++i) q[i] = (R)qd[i];
+    }
+    return q;
+  };
+  const auto qx = q0vec(g.Nx, g.fuse_x), qy = q0vec(g.Ny, g.fuse_y),
+             qz = q0vec(g.ln, g.fuse_z);
+  HIPCHK(hipMalloc(&p->P, P.size() * sizeof(cpx<R>)), "hipMalloc(P)");
+  HIPCHK(hipMemcpy(p->P, P.data(), P.size() * sizeof(cpx<R>), hipMemcpyHostToDevice), "copy P");
+  HIPCHK(hipMalloc((void **)&p->prop_a, pa.size() * sizeof(double)), "hipMalloc(prop)");
+  HIPCHK(hipMemcpy(p->prop_a, pa.data(), pa.size() * sizeof(double), hipMemcpyHostToDevice), "copy prop");
+  HIPCHK(hipMalloc(&p->aux, aux.size() * sizeof(PropAux<R>)), "hipMalloc(aux)");
+  HIPCHK(hipMemcpy(p->aux, aux.data(), aux.size() * sizeof(PropAux<R>), hipMemcpyHostToDevice), "copy aux");
+  HIPCHK(hipMalloc(&p->q0x, qx.size() * sizeof(R)), "hipMalloc(q0x)");
+  HIPCHK(hipMemcpy(p->q0x, qx.data(), qx.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0x");
+  HIPCHK(hipMalloc(&p->q0y, qy.size() * sizeof(R)), "hipMalloc(q0y)");
+  HIPCHK(hipMemcpy(p->q0y, qy.data(), qy.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0y");
+  HIPCHK(hipMalloc(&p->q0z, qz.size() * sizeof(R)), "hipMalloc(q0z)");
+  HIPCHK(hipMemcpy(p->q0z, qz.data(), qz.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0z");
+  HIPCHK(hipMalloc(&p->twx, (size_t)g.lNx * sizeof(cpx<R>)), "hipMalloc(twx)");
+  HIPCHK(hipMalloc(&p->twy, (size_t)g.lNy * sizeof(cpx<R>)), "hipMalloc(twy)");
+  HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twx, g.lNx, 0), "twiddles x");
+  HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twy, g.lNy, 0), "twiddles y");
+  HIPCHK(hipDeviceSynchronize(), "plan tables");
+  return ODT_OK;
+}
+
+int per_block(int n, size_t esz, int cap) {
+  size_t b = 65536 / ((size_t)2 * n * esz);
+  if (b < 1) b = 1;
+  if (b > 16) b = 16;
+  return std::max(1, std::min((int)b, cap));
+}
+
+void free_plan(odt_plan *p) {
+  void *ptrs[] = {p->twx, p->twy, p->P, p->aux, p->q0x, p->q0y, p->q0z,
+                  p->prop_a, p->weights, p->params};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  for (auto e : p->ev_rot) (void)hipEventDestroy(e);
+  if (p->ev_t0) (void)hipEventDestroy(p->ev_t0);
+  if (p->ev_t1) (void)hipEventDestroy(p->ev_t1);
+  delete p;
+}
+
+template <typename R, typename V>
+int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
+              hipStream_t st) {
+  const Geom &g = p->g;
+  cpx<R> *F1 = (cpx<R> *)(ws + p->off_F1);
+  cpx<R> *S = (cpx<R> *)(ws + p->off_S);
+  cpx<R> *T = (cpx<R> *)(ws + p->off_T);
+  V *slab = (V *)(ws + p->off_slab);
+  V *comp = p->off_comp ? (V *)(ws + p->off_comp) : nullptr;
+  V *out = (V *)d_out;
+  const cpx<R> *usin = (const cpx<R> *)d_usin;
+  const bool tilted = p->kind == ODT_KIND_3D_TILTED;
+  if (p->timing) HIPCHK(hipEventRecord(p->ev_t0, st), "event");
+  p->n_rot = 0;
+  for (int b0 = 0; b0 < p->A; b0 += p->NB) {
+    const int nb = std::min(p->NB, p->A - b0);
+    for (int c0 = b0; c0 < b0 + nb; c0 += p->NC) {
+      const int n = std::min(p->NC, b0 + nb - c0);
+      HIPCHK(odt::launch_fwd_rows<R>(usin, p->weights, F1, (const cpx<R> *)p->twx, g,
+                                     p->padval_re, p->padval_im, c0, n, p->nb_fr, st),
+             "k_fwd_rows");
+      HIPCHK(odt::launch_fwd_cols<R>(F1, S, (const cpx<R> *)p->twy, (const cpx<R> *)p->P,
+                                     g, n, p->nb_fc, st),
+             "k_fwd_cols");
+      HIPCHK(odt::launch_col_ifft<R>(S, T, (const cpx<R> *)p->twy, p->prop_a,
+                                     (const PropAux<R> *)p->aux, (const R *)p->q0z,
+                                     (const R *)p->q0y, g, n, p->nb_col, st),
+             "k_col_ifft");
+      HIPCHK((odt::launch_row_ifft<R, V>(T, slab, (const cpx<R> *)p->twx,
+                                         (const R *)p->q0x, g, n, p->nb_row, c0 - b0, st)),
+             "k_row_ifft");
+    }
+    if (p->explicit_iir) {
+      const long long ec = sizeof(V) / sizeof(R);
+      const long long Nx = g.Nx, Ny = g.Ny, ln = g.ln;
+      // along x
+      HIPCHK(odt::launch_iir<R>((R *)slab, nb * Ny * ln * ec, g.Nx, ec, 1, Nx * ec, ec,
+                                p->order, st), "iir x");
+      // along z
+      HIPCHK(odt::launch_iir<R>((R *)slab, nb * Ny * Nx * ec, g.ln, Nx * ec, 1,
+                                ln * Nx * ec, Nx * ec, p->order, st), "iir z");
+      if (tilted)
+        HIPCHK(odt::launch_iir<R>((R *)slab, nb * ln * Nx * ec, g.Ny, ln * Nx * ec, 1,
+                                  Ny * ln * Nx * ec, ln * Nx * ec, p->order, st),
+               "iir y");
+    }
+    const int first = b0 == 0, last = b0 + nb == p->A;
+    const double *prm = p->params + (size_t)b0 * p->pstride;
+    if (p->timing) {
+      if ((int)p->ev_rot.size() < 2 * (p->n_rot + 1)) {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0), "event");
+        HIPCHK(hipEventCreate(&e1), "event");
+        p->ev_rot.push_back(e0);
+        p->ev_rot.push_back(e1);
+      }
+      HIPCHK(hipEventRecord(p->ev_rot[2 * p->n_rot], st), "event");
+    }
+    if (tilted)
+      HIPCHK((odt::launch_affine_acc<R, V>(p->order, slab, nb, prm, out, comp, g.Ny,
+                                           g.ln, g.Nx, first, last, st)),
+             "k_affine_acc");
+    else
+      HIPCHK((odt::launch_rotate_acc<R, V>(p->order, slab, nb, prm, out, comp, g.Ny,
+                                           g.ln, g.Nx, first, last, st)),
+             "k_rotate_acc");
+    if (p->timing) HIPCHK(hipEventRecord(p->ev_rot[2 * p->n_rot + 1], st), "event");
+    p->n_rot++;
+  }
+  if (p->timing) HIPCHK(hipEventRecord(p->ev_t1, st), "event");
+  return ODT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *odt_last_error(void) { return g_err.c_str(); }
+
+int odt_version(void) { return 1; }
+
+int odt_plan_create(const odt_desc *d, odt_plan **out) {
+  if (!d || !out) return fail(ODT_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (d->kind < 0 || d->kind > 2) return fail(ODT_ERR_INVALID, "unknown kind");
+  if (d->precision != ODT_FP32 && d->precision != ODT_FP64)
+    return fail(ODT_ERR_INVALID, "precision must be ODT_FP32 or ODT_FP64");
+  if (d->A < 1 || d->Ny < 1 || d->Nx < 1)
+    return fail(ODT_ERR_INVALID, "A, Ny, Nx must be positive");
+  if (d->lNy < d->Ny || d->lNx < d->Nx || d->padyl < 0 || d->padxl < 0 ||
+      d->padyl + d->Ny > d->lNy || d->padxl + d->Nx > d->lNx)
+    return fail(ODT_ERR_INVALID, "inconsistent padding");
+  if (d->pad_mode < 0 || d->pad_mode > 2)
+    return fail(ODT_ERR_INVALID, "pad_mode");
+  if (d->pad_mode == ODT_PAD_NONE && (d->lNy != d->Ny || d->lNx != d->Nx))
+    return fail(ODT_ERR_INVALID, "pad_mode NONE needs lNy == Ny and lNx == Nx");
+  if (d->intp_order < 0 || d->intp_order > 5)
+    return fail(ODT_ERR_INVALID, "intp_order must be in 0..5");
+  const int want_stride = d->kind == ODT_KIND_3D_TILTED ? 12 : 6;
+  if (d->angle_param_stride != want_stride)
+    return fail(ODT_ERR_INVALID, "angle_param_stride must be 6 (planar) or 12 (affine)");
+  if (!d->weights || !d->angle_params)
+    return fail(ODT_ERR_INVALID, "weights and angle_params are required");
+  if (!(d->km > 0)) return fail(ODT_ERR_INVALID, "km must be positive");
+  if (d->kind == ODT_KIND_2D && (d->Ny != 1 || d->lNy != 1))
+    return fail(ODT_ERR_INVALID, "2D plans need Ny == lNy == 1");
+  const size_t esz = d->precision == ODT_FP64 ? 16 : 8;
+  const size_t max_lds = 160 * 1024;
+  if (2 * (size_t)d->lNx * esz > max_lds || 2 * (size_t)d->lNy * esz > max_lds)
+    return fail(ODT_ERR_UNSUPPORTED, "transform length exceeds the LDS capacity");
+
+  odt_plan *p = new odt_plan();
+  p->kind = d->kind;
+  p->precision = d->precision;
+  p->order = d->intp_order;
+  p->onlyreal = d->onlyreal ? 1 : 0;
+  p->flags = d->flags;
+  p->pstride = want_stride;
+  p->A = d->A;
+  p->padval_re = d->padval_re;
+  p->padval_im = d->padval_im;
+  Geom &g = p->g;
+  std::memset(&g, 0, sizeof(g));
+  g.A = d->A;
+  g.Ny = d->Ny;
+  g.Nx = d->Nx;
+  g.ln = d->Nx;
+  g.lNy = d->lNy;
+  g.lNx = d->lNx;
+  g.logNy = ilog2_exact(d->lNy);
+  g.logNx = ilog2_exact(d->lNx);
+  g.padyl = d->padyl;
+  g.padxl = d->padxl;
+  g.pad_mode = d->pad_mode;
+  g.z0 = d->z0;
+  support(g.lNx, d->km, &g.jx0, &g.Lx);
+  support(g.lNy, d->km, &g.jy0, &g.Ly);
+  const bool pf = p->order >= 2;
+  const bool want_iir = (d->flags & ODT_FLAG_IIR_PREFILTER) != 0;
+  const bool fusable = p->order == 2 || p->order == 3;
+  p->explicit_iir = pf && (want_iir || !fusable);
+  const bool fused = pf && !p->explicit_iir;
+  g.fuse_x = fused && g.Nx > 1;
+  g.fuse_z = fused && g.ln > 1;
+  g.fuse_y = fused && d->kind == ODT_KIND_3D_TILTED && g.Ny > 1;
+
+  p->esz_c = esz;
+  p->esz_v = p->onlyreal ? esz / 2 : esz;
+  const size_t slab_angle = (size_t)g.Ny * g.ln * g.Nx * p->esz_v;
+  size_t budget = d->slab_budget_bytes > 0 ? (size_t)d->slab_budget_bytes
+                                           : ((size_t)8 << 30);
+  p->NB = (int)std::max<size_t>(1, std::min<size_t>(budget / slab_angle, (size_t)d->A));
+  const size_t t_angle = (size_t)g.ln * g.Lx * g.Ny * esz;
+  const size_t t_budget = (size_t)1 << 30;
+  p->NC = (int)std::max<size_t>(1, std::min<size_t>(t_budget / t_angle, (size_t)p->NB));
+  p->NC = std::min(p->NC, 64);
+  p->nb_fr = per_block(g.lNx, esz, g.lNy);
+  p->nb_fc = per_block(g.lNy, esz, g.Lx);
+  p->nb_col = per_block(g.lNy, esz, g.ln);
+  p->nb_row = per_block(g.lNx, esz, g.Ny);
+
+  size_t off = 0;
+  p->off_F1 = off;
+  off = align256(off + (size_t)p->NC * g.Lx * g.lNy * esz);
+  p->off_S = off;
+  off = align256(off + (size_t)p->NC * g.Lx * g.Ly * esz);
+  p->off_T = off;
+  off = align256(off + (size_t)p->NC * g.ln * g.Lx * g.Ny * esz);
+  p->off_slab = off;
+  off = align256(off + (size_t)p->NB * slab_angle);
+  p->off_comp = 0;
+  if (p->NB < d->A) {
+    p->off_comp = off;
+    off = align256(off + slab_angle);
+  }
+  p->ws_bytes = off;
+
+  int rc;
+  hipError_t e;
+  if ((e = hipMalloc((void **)&p->weights, sizeof(double) * d->A)) != hipSuccess ||
+      (e = hipMemcpy(p->weights, d->weights, sizeof(double) * d->A,
+                     hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMalloc((void **)&p->params, sizeof(double) * d->A * want_stride)) !=
+          hipSuccess ||
+      (e = hipMemcpy(p->params, d->angle_params, sizeof(double) * d->A * want_stride,
+                     hipMemcpyHostToDevice)) != hipSuccess) {
+    free_plan(p);
+    return hip_fail(e, "plan upload");
+  }
+  rc = d->precision == ODT_FP64 ? upload_tables<double>(p, d) : upload_tables<float>(p, d);
+  if (rc != ODT_OK) {
+    std::string keep = g_err;
+    free_plan(p);
+    g_err = keep;
+    return rc;
+  }
+  if ((e = hipEventCreate(&p->ev_t0)) != hipSuccess ||
+      (e = hipEventCreate(&p->ev_t1)) != hipSuccess) {
+    free_plan(p);
+    return hip_fail(e, "event");
+  }
+  *out = p;
+  return ODT_OK;
+}
+
+int odt_plan_workspace_size(const odt_plan *p, size_t *bytes) {
+  if (!p || !bytes) return fail(ODT_ERR_INVALID, "null argument");
+  *bytes = p->ws_bytes;
+  return ODT_OK;
+}
+
+int odt_plan_execute(odt_plan *p, const void *d_usin, void *d_out, void *ws,
+                     size_t ws_bytes, void *stream) {
+  if (!p || !d_usin || !d_out) return fail(ODT_ERR_INVALID, "null argument");
+  if (ws_bytes < p->ws_bytes || (!ws && p->ws_bytes))
+    return fail(ODT_ERR_WORKSPACE, "workspace too small: need " +
+                                       std::to_string(p->ws_bytes) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  char *w = (char *)ws;
+  if (p->precision == ODT_FP64)
+    return p->onlyreal ? execute_t<double, double>(p, d_usin, d_out, w, st)
+                       : execute_t<double, cpx<double>>(p, d_usin, d_out, w, st);
+  return p->onlyreal ? execute_t<float, float>(p, d_usin, d_out, w, st)
+                     : execute_t<float, cpx<float>>(p, d_usin, d_out, w, st);
+}
+
+int odt_plan_destroy(odt_plan *p) {
+  if (p) free_plan(p);
+  return ODT_OK;
+}
+
+int odt_plan_enable_timing(odt_plan *p, int enable) {
+  if (!p) return fail(ODT_ERR_INVALID, "null plan");
+  p->timing = enable != 0;
+  return ODT_OK;
+}
+
+int odt_plan_last_timing(const odt_plan *p, double *rot_ms, double *filt_ms,
+                         double *total_ms, int *launches) {
+  if (!p) return fail(ODT_ERR_INVALID, "null plan");
+  if (!p->timing) return fail(ODT_ERR_INVALID, "timing not enabled");
+  HIPCHK(hipEventSynchronize(p->ev_t1), "event sync");
+  float t = 0;
+  double rot = 0;
+  for (int i = 0; i < p->n_rot; ++i) {
+    HIPCHK(hipEventElapsedTime(&t, p->ev_rot[2 * i], p->ev_rot[2 * i + 1]), "elapsed");
+    rot += t;
+  }
+  HIPCHK(hipEventElapsedTime(&t, p->ev_t0, p->ev_t1), "elapsed");
+  if (rot_ms) *rot_ms = rot;
+  if (total_ms) *total_ms = t;
+  if (filt_ms) *filt_ms = t - rot;
+  if (launches) *launches = p->n_rot;
+  return ODT_OK;
+}
+
+int odt_fft_c2c(int precision, int64_t batch, int32_t n, int32_t direction,
+                const void *d_in, void *d_out, void *stream) {
+  if (precision != ODT_FP32 && precision != ODT_FP64)
+    return fail(ODT_ERR_INVALID, "precision");
+  if (batch < 0 || n < 1 || (direction != 1 && direction != -1) || !d_in || !d_out)
+    return fail(ODT_ERR_INVALID, "bad fft arguments");
+  const size_t esz = precision == ODT_FP64 ? 16 : 8;
+  if (2 * (size_t)n * esz > 160 * 1024)
+    return fail(ODT_ERR_UNSUPPORTED, "transform length exceeds the LDS capacity");
+  if (batch == 0) return ODT_OK;
+  hipStream_t st = (hipStream_t)stream;
+  void *tw = nullptr;
+  HIPCHK(hipMalloc(&tw, (size_t)n * esz), "hipMalloc(tw)");
+  const int nb = per_block(n, esz, (int)std::min<int64_t>(batch, 16));
+  const int logn = ilog2_exact(n);
+  hipError_t e;
+  if (precision == ODT_FP64) {
+    e = odt::launch_twiddles<double>((cpx<double> *)tw, n, st);
+    if (e == hipSuccess)
+      e = odt::launch_fft_batch<double>((const cpx<double> *)d_in, (cpx<double> *)d_out,
+                                        batch, n, logn, direction,
+                                        (const cpx<double> *)tw, nb, st);
+  } else {
+    e = odt::launch_twiddles<float>((cpx<float> *)tw, n, st);
+    if (e == hipSuccess)
+      e = odt::launch_fft_batch<float>((const cpx<float> *)d_in, (cpx<float> *)d_out,
+                                       batch, n, logn, direction,
+                                       (const cpx<float> *)tw, nb, st);
+  }
+  hipError_t e2 = hipStreamSynchronize(st);
+  (void)hipFree(tw);
+  if (e != hipSuccess) return hip_fail(e, "k_fft_batch");
+  if (e2 != hipSuccess) return hip_fail(e2, "k_fft_batch sync");
+  return ODT_OK;
+}
+
+int odt_spline_prefilter(int precision, int order, int64_t nlines, int32_t n,
+                         int64_t line_stride, int64_t elem_stride, void *d_data,
+                         void *stream) {
+  if (precision != ODT_FP32 && precision != ODT_FP64)
+    return fail(ODT_ERR_INVALID, "precision");
+  if (order < 2 || order > 5) return fail(ODT_ERR_INVALID, "order must be 2..5");
+  if (nlines < 0 || n < 1 || !d_data) return fail(ODT_ERR_INVALID, "bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e =
+      precision == ODT_FP64
+          ? odt::launch_iir<double>((double *)d_data, nlines, n, 1, 0, line_stride,
+                                    elem_stride, order, st)
+          : odt::launch_iir<float>((float *)d_data, nlines, n, 1, 0, line_stride,
+                                   elem_stride, order, st);
+  if (e != hipSuccess) return hip_fail(e, "k_iir");
+  return ODT_OK;
+}
+
+int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
+                          int32_t nz, int32_t nx, const void *d_coef,
+                          const double *params, void *d_out, int accumulate,
+                          void *stream) {
+  if (precision != ODT_FP32 && precision != ODT_FP64)
+    return fail(ODT_ERR_INVALID, "precision");
+  if (order < 0 || order > 5) return fail(ODT_ERR_INVALID, "order must be 0..5");
+  if (nang < 1 || ny < 1 || nz < 1 || nx < 1 || !d_coef || !params || !d_out)
+    return fail(ODT_ERR_INVALID, "bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  double *dp = nullptr;
+  HIPCHK(hipMalloc((void **)&dp, sizeof(double) * 6 * nang), "hipMalloc(params)");
+  hipError_t e = hipMemcpyAsync(dp, params, sizeof(double) * 6 * nang,
+                                hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    const int first = accumulate ? 0 : 1;
+    e = precision == ODT_FP64
+            ? odt::launch_rotate_acc<double, double>(order, (const double *)d_coef, nang,
+                                                     dp, (double *)d_out, nullptr, ny,
+                                                     nz, nx, first, 1, st)
+            : odt::launch_rotate_acc<float, float>(order, (const float *)d_coef, nang,
+                                                   dp, (float *)d_out, nullptr, ny, nz,
+                                                   nx, first, 1, st);
+  }
+  hipError_t e2 = hipStreamSynchronize(st);
+  (void)hipFree(dp);
+  if (e != hipSuccess) return hip_fail(e, "k_rotate_acc");
+  if (e2 != hipSuccess) return hip_fail(e2, "k_rotate_acc sync");
+  return ODT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+"""Pins of the CPU oracle (oracle/) against the reference's own evidence:
+golden fixtures of ODTbrain's tests, closed forms and known answers, and
+scipy.ndimage for the B-spline restatement.  CPU only."""
+import ctypes
+import multiprocessing as mp
+
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+import scipy.special
+
+from common import (create_test_sino_2d, create_test_sino_3d, cutout,
+                    get_test_parameter_set, load_golden)
+from oracle import bspline as B
+from oracle import odt_oracle as O
+
+
+def test_oracle_3d_backprop_phase_fixture():
+    """tests/test_alg3d_bpp.py:30-42 (padval=0 ramp, complex)."""
+    sino, angles = create_test_sino_3d()
+    r = [cutout(O.backpropagate_3d(sino, angles, padval=0, dtype=float, **p))
+         for p in get_test_parameter_set(2)]
+    data = np.array(r).flatten().view(float)
+    assert np.allclose(data, load_golden("alg3d_bpp__test_3d_backprop_phase"))
+
+
+def test_oracle_3d_backprop_nopadreal_fixture():
+    """tests/test_alg3d_bpp.py:45-62 (no padding, onlyreal)."""
+    sino, angles = create_test_sino_3d()
+    r = [cutout(O.backpropagate_3d(sino, angles, padding=(False, False),
+                                   dtype=float, onlyreal=True, **p))
+         for p in get_test_parameter_set(2)]
+    data = np.array(r).flatten().view(float)
+    assert np.allclose(data,
+                       load_golden("alg3d_bpp__test_3d_backprop_nopadreal"))
+
+
+def test_oracle_mprotate_fixture_pool():
+    """tests/test_alg3d_bpp.py:116-134: arange(20^3) rotated by 2 degrees at
+    order 2 over y in [0, 10) through a process pool on a shared array."""
+    ln2, ln = 20, 10
+    shared = mp.RawArray(ctypes.c_double, ln2 ** 3)
+    arr = np.frombuffer(shared).reshape(ln2, ln2, ln2)
+    arr[:] = np.arange(ln2 ** 3).reshape(ln2, ln2, ln2)
+    pool = mp.get_context("fork").Pool(
+        3, initializer=O._pool_init,
+        initargs=(shared, (ln2, ln2, ln2), np.dtype(float)))
+    try:
+        O.mprotate(2, ln, pool, 2, 3)
+    finally:
+        pool.terminate()
+        pool.join()
+    assert np.allclose(arr.flatten(), load_golden("alg3d_bpp__test_3d_mprotate"))
+
+
+@pytest.mark.parametrize("name,ampl", [("alg2d_bpp__test_2d_backprop_phase",
+                                        (1.0, 1.0)),
+                                       ("alg2d_bpp__test_2d_backprop_full",
+                                        (0.9, 1.1))])
+def test_oracle_2d_fixtures(name, ampl):
+    """tests/test_alg2d_bpp.py:13-36."""
+    sino, angles = create_test_sino_2d(ampl_range=ampl)
+    r = [cutout(O.backpropagate_2d(sino, angles, **p))
+         for p in get_test_parameter_set(2)]
+    assert np.allclose(np.array(r).flatten().view(float), load_golden(name))
+
+
+def test_oracle_tilted_equals_plain_for_y_axis():
+    """tests/test_alg3d_bppt.py:10-50 (both pinned through the fixtures)."""
+    sino, angles = create_test_sino_3d()
+    for p in get_test_parameter_set(2):
+        a = cutout(O.backpropagate_3d(sino, angles, padval=0, dtype=float,
+                                      onlyreal=True, **p))
+        b = cutout(O.backpropagate_3d_tilted(sino, angles, padval=0,
+                                             dtype=float, onlyreal=True, **p))
+        assert np.allclose(a, b)
+
+
+def test_oracle_weights_closed_forms():
+    """tests/test_util.py:6-32."""
+    w = O.compute_angle_weights_1d(np.linspace(0, np.pi, 18, endpoint=False))
+    assert np.allclose(w, 1, rtol=0, atol=1e-14)
+    w = O.compute_angle_weights_1d(np.linspace(0, np.pi, 18, endpoint=True))
+    assert np.allclose(w[1:-1], 18 / 17, rtol=0, atol=1e-14)
+    assert np.allclose(w[[0, -1]], 18 / 17 / 2, rtol=0, atol=1e-14)
+    a0 = np.linspace(0, np.pi, 14, endpoint=False)
+    a1 = np.roll(a0, -3)
+    a = np.roll(np.concatenate((np.ones(4) * a1[0], a1)), 3)
+    w = O.compute_angle_weights_1d(a)
+    assert np.allclose(np.mean(w), 1, rtol=0, atol=1e-14)
+    assert np.allclose(w[:3], 18 / 14, rtol=0, atol=1e-14)
+    assert np.allclose(w[8:], 18 / 14, rtol=0, atol=1e-14)
+    assert np.allclose(w[3:8], 18 / 14 / 5, rtol=0, atol=1e-14)
+
+
+def test_oracle_sphere_points_known_answers():
+    """tests/test_spherecoords_from_angles_and_axis.py:8-32."""
+    angles = np.array([0, np.pi / 2, np.pi])
+    axes = [[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 1, 1], [1, 0, 1], [1, 1, 1]]
+    res = [O.sphere_points_from_angles_and_tilt(angles, ax) for ax in axes]
+    s2 = 1 / np.sqrt(2)
+    correct = np.array([[[1, 0, 0], [1, 0, 0], [1, 0, 0]],
+                        [[0, 0, 1], [1, 0, 0], [0, 0, -1]],
+                        [[0, 0, 1], [0, 0, 1], [0, 0, 1]],
+                        [[0, 0, 1], [s2, .5, .5], [0, 1, 0]],
+                        [[s2, 0, s2], [s2, 0, s2], [s2, 0, s2]],
+                        [[s2, 0, s2],
+                         [0.87965281125489458, s2 / 3 * 2,
+                          0.063156230327168605],
+                         [s2 / 3, s2 / 3 * 4, s2 / 3]]])
+    assert np.allclose(correct, np.array(res))
+
+
+def test_oracle_rotation_matrices_known_answers():
+    """tests/test_rotation_matrices.py:8-102."""
+    sq2 = np.sqrt(2)
+    r = O.rotate_points_to_axis([[1, 2, 3]], [1, 2, 3])
+    assert r[0][0] < 1e-14 and r[0][2] < 1e-14
+    r = O.rotate_points_to_axis([[0, 0, 1], [1, 0, 0], [1, 1, 0]], [0, 1, 1])
+    assert np.allclose(r, [[0, 1 / sq2, 1 / sq2], [1, 0, 0],
+                           [1, 1 / sq2, -1 / sq2]])
+    r = O.rotate_points_to_axis([[0, 0, 1], [1, 0, 0], [0, -1, 0]], [1, 0, 1])
+    assert np.allclose(r, [[-.5, 1 / sq2, .5], [.5, 1 / sq2, -.5],
+                           [1 / sq2, 0, 1 / sq2]])
+    r = O.rotate_points_to_axis([[0, 0, 1], [1, 0, 1], [1, -1, 0]], [1, 0, 0])
+    assert np.allclose(r, [[0, 0, 1], [0, 1, 1], [1, 1, 0]])
+    r = O.rotate_points_to_axis([[0, 0, 1], [1, 0, 0], [1, -1, 0]], [0, -1, 0])
+    assert np.allclose(r, [[0, 0, -1], [1, 0, 0], [1, 1, 0]])
+    m = O.rotation_matrix_from_point([0, 1, 1])
+    assert np.allclose(m @ [0, 0, 1], [0, 1 / sq2, 1 / sq2])
+    assert np.allclose(m @ [1, 0, 0], [1, 0, 0])
+    m = O.rotation_matrix_from_point([-1, 1, 0])
+    assert np.allclose(m @ [1, 0, 0], [0, 0, -1])
+    assert np.allclose(m @ [0, 1 / sq2, -1 / sq2], [-1, 0, 0])
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+def test_bspline_prefilter_matches_scipy(order):
+    rng = np.random.default_rng(order)
+    x = rng.standard_normal((6, 11))
+    for ax in (0, 1):
+        ref = ndi.spline_filter1d(x, order, axis=ax, mode="mirror")
+        assert np.allclose(B.spline_filter1d(x, order, ax), ref,
+                           rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 4, 5])
+def test_bspline_rotate_matches_scipy(order):
+    """Including the exact out-of-bounds set at 90-degree multiples."""
+    rng = np.random.default_rng(10 + order)
+    for ang in (2.0, 90.0, -270.0, 33.3, 180.0, -17.5):
+        img = rng.standard_normal((9, 12))
+        got = B.rotate_plane(img, ang, order, scipy.special.cosdg,
+                             scipy.special.sindg)
+        ref = ndi.rotate(img, ang, reshape=False, order=order,
+                         mode="constant", cval=0)
+        assert np.array_equal(got == 0, ref == 0)
+        assert np.allclose(got, ref, rtol=0, atol=1e-11)
+
+
+def test_bspline_restatement_reproduces_mprotate_fixture():
+    ln2 = 20
+    arr = np.arange(ln2 ** 3, dtype=float).reshape(ln2, ln2, ln2)
+    for y in range(10):
+        arr[:, y, :] = B.rotate_plane(arr[:, y, :], -2.0, 2,
+                                      scipy.special.cosdg,
+                                      scipy.special.sindg)
+    assert np.allclose(arr.flatten(), load_golden("alg3d_bpp__test_3d_mprotate"))
+
+
+def test_oracle_planted_mistake_is_caught():
+    """A dropped term in the prefactor (the lD phase) must break the fixture
+    check, showing the fixture pins more than the formula's shape."""
+    sino, angles = create_test_sino_3d()
+    p = get_test_parameter_set(2)[1]           # lD = pi
+    good = cutout(O.backpropagate_3d(sino, angles, padval=0, dtype=float, **p))
+    bad = cutout(O.backpropagate_3d(sino, angles, padval=0, dtype=float,
+                                    **dict(p, lD=0)))
+    ref = load_golden("alg3d_bpp__test_3d_backprop_phase").view(complex)
+    ref = ref[ref.size // 2:].reshape(good.shape)
+    assert np.allclose(good, ref)
+    assert not np.allclose(bad, ref)
