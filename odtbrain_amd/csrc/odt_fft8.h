@@ -1,0 +1,166 @@
+// Register-resident Stockham FFT for power-of-two lengths 8..4096.
+//
+// T = N/8 threads cooperate on one transform; each thread holds 8 elements in
+// registers.  Stages are radix 8 (plus one radix-2 or radix-4 stage when
+// log2 N is not a multiple of 3).  Stage s (radix R, L = product of earlier
+// radices) computes, for butterfly b (k = b mod L), the R inputs
+// x[b + r*N/R] times the twiddles w_{LR}^{r k}, an R-point DFT, and writes
+// y[(b - k) R + k + q L].  Thread j owns butterflies b = j + i*T.
+// Input and output use the same "strided" layout: register m of thread j is
+// element j + m*T, so global loads and stores are coalesced across threads.
+// Between stages the data goes through LDS (ping-pong buffers, one barrier
+// per exchange, one padding slot per 16 elements against bank conflicts).
+#pragma once
+
+#include "odt_device.h"
+
+namespace odt {
+
+__host__ __device__ constexpr int ilog2c(int n) {
+  return n <= 1 ? 0 : 1 + ilog2c(n >> 1);
+}
+
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_padded(int n) { return n + (n >> 4) + 1; }
+
+// e^{dir * i * pi/4} etc. for the radix-8 butterfly
+template <typename F, int DIR>
+__device__ __forceinline__ cpx<F> mul_mi(cpx<F> a) {  // a * (dir*i)
+  return DIR > 0 ? mk<F>(-a.im, a.re) : mk<F>(a.im, -a.re);
+}
+
+template <typename F, int DIR>
+__device__ __forceinline__ void dft2(cpx<F> &a0, cpx<F> &a1) {
+  const cpx<F> t = a0 - a1;
+  a0 = a0 + a1;
+  a1 = t;
+}
+
+template <typename F, int DIR>
+__device__ __forceinline__ void dft4(cpx<F> &x0, cpx<F> &x1, cpx<F> &x2,
+                                     cpx<F> &x3) {
+  const cpx<F> t0 = x0 + x2, t1 = x0 - x2;
+  const cpx<F> t2 = x1 + x3, t3 = mul_mi<F, DIR>(x1 - x3);
+  x0 = t0 + t2;
+  x2 = t0 - t2;
+  x1 = t1 + t3;
+  x3 = t1 - t3;
+}
+
+template <typename F, int DIR>
+__device__ __forceinline__ void dft8(cpx<F> *a) {
+  // even/odd split: E = DFT4(a0,a2,a4,a6), O = DFT4(a1,a3,a5,a7)
+  cpx<F> e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  cpx<F> o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<F, DIR>(e0, e1, e2, e3);
+  dft4<F, DIR>(o0, o1, o2, o3);
+  const F h = (F)0.70710678118654752440;
+  // w8^1 = (1 + dir*i)/sqrt2, w8^2 = dir*i, w8^3 = (-1 + dir*i)/sqrt2
+  const cpx<F> w1o1 = DIR > 0 ? mk<F>(h * (o1.re - o1.im), h * (o1.re + o1.im))
+                              : mk<F>(h * (o1.re + o1.im), h * (o1.im - o1.re));
+  const cpx<F> w2o2 = mul_mi<F, DIR>(o2);
+  const cpx<F> w3o3 = DIR > 0
+                          ? mk<F>(-h * (o3.re + o3.im), h * (o3.re - o3.im))
+                          : mk<F>(h * (o3.im - o3.re), -h * (o3.re + o3.im));
+  a[0] = e0 + o0;
+  a[4] = e0 - o0;
+  a[1] = e1 + w1o1;
+  a[5] = e1 - w1o1;
+  a[2] = e2 + w2o2;
+  a[6] = e2 - w2o2;
+  a[3] = e3 + w3o3;
+  a[7] = e3 - w3o3;
+}
+
+template <typename F, int R, int DIR>
+__device__ __forceinline__ void dft_r(cpx<F> *a) {
+  if (R == 8) {
+    dft8<F, DIR>(a);
+  } else if (R == 4) {
+    dft4<F, DIR>(a[0], a[1], a[2], a[3]);
+  } else {
+    dft2<F, DIR>(a[0], a[1]);
+  }
+}
+
+template <typename F, int N, int DIR>
+struct Fft8 {
+  static constexpr int T = N / 8;
+  static constexpr int LOG = ilog2c(N);
+  static constexpr int N8 = LOG / 3;
+  static constexpr int REM = LOG % 3;
+  static constexpr int NS = N8 + (REM ? 1 : 0);
+  static constexpr int PADN = lds_padded(N);
+  static __host__ __device__ constexpr int radix(int s) {
+    return s < N8 ? 8 : (REM == 1 ? 2 : 4);
+  }
+
+  template <int S, int L>
+  __device__ static __forceinline__ void stage(cpx<F> *v, cpx<F> *buf0,
+                                               cpx<F> *buf1,
+                                               const cpx<F> *__restrict__ tw,
+                                               int j) {
+    constexpr int R = radix(S);
+    constexpr int NB = 8 / R;
+    constexpr bool FIRST = S == 0;
+    constexpr bool LAST = S == NS - 1;
+    cpx<F> *src = (S & 1) ? buf0 : buf1;  // written by stage S-1
+    cpx<F> *dst = (S & 1) ? buf1 : buf0;
+    cpx<F> a[8];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        a[i * R + r] = FIRST ? v[i + r * NB] : src[lds_pad(j + i * T + r * (N / R))];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = j + i * T;
+      const int k = b & (L - 1);
+      if (L > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cpx<F> w = conj_if(tw[r * k * (N / (L * R))], DIR < 0);
+          a[i * R + r] = a[i * R + r] * w;
+        }
+      }
+      dft_r<F, R, DIR>(a + i * R);
+    }
+    if (LAST) {
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[i + q * NB] = a[i * R + q];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int b = j + i * T;
+        const int k = b & (L - 1);
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          dst[lds_pad((b - k) * R + k + q * L)] = a[i * R + q];
+      }
+      __syncthreads();
+    }
+  }
+
+  template <int S, int L>
+  __device__ static __forceinline__ void stages(cpx<F> *v, cpx<F> *b0,
+                                                cpx<F> *b1, const cpx<F> *tw,
+                                                int j) {
+    stage<S, L>(v, b0, b1, tw, j);
+    if constexpr (S + 1 < NS) stages<S + 1, L * radix(S)>(v, b0, b1, tw, j);
+  }
+
+  // v[m] = x[j + m*T] on entry, X[j + m*T] on exit.  buf0/buf1: this
+  // transform's two LDS scratch buffers of PADN elements.  Must be called
+  // by all threads of the block (it contains barriers).
+  __device__ static __forceinline__ void run(cpx<F> *v, cpx<F> *buf0,
+                                             cpx<F> *buf1, const cpx<F> *tw,
+                                             int j) {
+    // the previous call's last stage may still be reading the buffers
+    if (NS > 1) __syncthreads();
+    stages<0, 1>(v, buf0, buf1, tw, j);
+  }
+};
+
+}  // namespace odt

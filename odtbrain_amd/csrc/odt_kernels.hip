@@ -20,7 +20,10 @@
 // k_iir is the explicit separable prefilter (used when the prefilter is not
 // folded into the transforms), k_fft_batch backs the standalone transform
 // entry point.
+#include <algorithm>
+
 #include "odt_device.h"
+#include "odt_fft8.h"
 #include "odt_launch.h"
 
 namespace odt {
@@ -241,6 +244,164 @@ template <typename R>
 __device__ __forceinline__ void store_val(R *p, cpx<R> v) { *p = v.re; }
 template <typename R>
 __device__ __forceinline__ void store_val(cpx<R> *p, cpx<R> v) { *p = v; }
+
+// Signed frequency index of FFT bin `bin` (numpy.fft.fftfreq order).
+template <int N>
+__device__ __forceinline__ int freq_of_bin(int bin) {
+  return bin < (N + 1) / 2 ? bin : bin - N;
+}
+
+// Fused-prefilter boundary terms of one transform: the owners of the four
+// outputs Y[p-1], Y[p+1], Y[p+n-2], Y[p+n] (indices mod N, p = crop start,
+// n = crop length) publish them in `corr`; returns
+// R0 = b1 (Y[p-1] - Y[p+1]) and R1 = b1 (Y[p+n] - Y[p+n-2]).
+template <typename R, int N, int TT>
+__device__ __forceinline__ void boundary_terms(const cpx<R> *v, int j, int p,
+                                               int n, R b1, cpx<R> *corr,
+                                               cpx<R> &R0, cpx<R> &R1) {
+  const int sb[4] = {(p - 1 + N) & (N - 1), (p + 1) & (N - 1),
+                     (p + n - 2 + N) & (N - 1), (p + n) & (N - 1)};
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int bin = j + m * TT;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (bin == sb[i]) corr[i] = v[m];
+  }
+  __syncthreads();
+  R0 = b1 * (corr[0] - corr[1]);
+  R1 = b1 * (corr[3] - corr[2]);
+}
+
+// Column stage, register radix-8 engine (power-of-two lNy = N).
+// Block: tpb = blockDim/(N/8) transforms; column c, slices k0..k0+kb-1.
+template <typename R, int N>
+__global__ void __launch_bounds__(512)
+    k_col_fft8(const cpx<R> *__restrict__ S, cpx<R> *__restrict__ T,
+               const cpx<R> *__restrict__ twy,
+               const double *__restrict__ prop_a,
+               const PropAux<R> *__restrict__ aux,
+               const R *__restrict__ q0z, const R *__restrict__ q0y, Geom g,
+               int kb) {
+  using FFT = Fft8<R, N, +1>;
+  constexpr int TT = FFT::T;
+  const int tpb = blockDim.x / TT;
+  const int t = threadIdx.x / TT, j = threadIdx.x - t * TT;
+  cpx<R> *lds = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *b0 = lds + 2 * t * FFT::PADN, *b1 = b0 + FFT::PADN;
+  cpx<R> *corr = lds + 2 * tpb * FFT::PADN + 4 * t;
+  const int c = blockIdx.x, k0 = blockIdx.y * kb, ai = blockIdx.z;
+  const int ln = g.ln, ny = g.Ny;
+  const int kend = min(k0 + kb, ln);
+  cpx<R> sv[8];
+  double pa[8];
+  PropAux<R> ax[8];
+  unsigned inside = 0;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int cy = freq_of_bin<N>(j + m * TT) - g.jy0;
+    sv[m] = mk<R>(0, 0);
+    pa[m] = 0.0;
+    if (cy >= 0 && cy < g.Ly) {
+      inside |= 1u << m;
+      const size_t tix = (size_t)c * g.Ly + cy;
+      sv[m] = S[((size_t)ai * g.Lx + c) * g.Ly + cy];
+      pa[m] = prop_a[tix];
+      if (g.fuse_z) ax[m] = aux[tix];
+    }
+  }
+  const int iters = (kb + tpb - 1) / tpb;
+  for (int it = 0; it < iters; ++it) {
+    const int k = k0 + it * tpb + t;
+    const bool valid = k < kend;
+    const double z = g.z0 + (double)k;
+    cpx<R> v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      v[m] = mk<R>(0, 0);
+      if (valid && ((inside >> m) & 1u)) {
+        const cpx<R> e = phasor(pa[m] * z, R(0));
+        cpx<R> gk = e;
+        if (g.fuse_z)
+          gk = ax[m].ib * e + q0z[k] * ax[m].r0 + q0z[ln - 1 - k] * ax[m].r1;
+        v[m] = sv[m] * gk;
+      }
+    }
+    FFT::run(v, b0, b1, twy, j);
+    cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
+    if (g.fuse_y)
+      boundary_terms<R, N, TT>(v, j, g.padyl, ny, (R)g.b1, corr, R0, R1);
+    if (valid) {
+      cpx<R> *dst = T + (((size_t)ai * ln + k) * g.Lx + c) * ny;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int y = j + m * TT - g.padyl;
+        if (y >= 0 && y < ny) {
+          cpx<R> val = v[m];
+          if (g.fuse_y) val = val + q0y[y] * R0 + q0y[ny - 1 - y] * R1;
+          dst[y] = val;
+        }
+      }
+    }
+  }
+}
+
+// Row stage, register radix-8 engine (power-of-two lNx = N).  The block
+// stages the support columns of rows y0..y0+rb-1 of slice k in LDS
+// (coalesced reads of T[a][k][c][y]), then transforms them tpb at a time.
+template <typename R, typename V, int N>
+__global__ void __launch_bounds__(512)
+    k_row_fft8(const cpx<R> *__restrict__ T, V *__restrict__ slab,
+               const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
+               Geom g, int rb, int slab0) {
+  using FFT = Fft8<R, N, +1>;
+  constexpr int TT = FFT::T;
+  const int tpb = blockDim.x / TT;
+  const int t = threadIdx.x / TT, j = threadIdx.x - t * TT;
+  cpx<R> *lds = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *b0 = lds + 2 * t * FFT::PADN, *b1 = b0 + FFT::PADN;
+  cpx<R> *corr = lds + 2 * tpb * FFT::PADN + 4 * t;
+  cpx<R> *tile = lds + 2 * tpb * FFT::PADN + 4 * tpb;
+  const int y0 = blockIdx.x * rb, k = blockIdx.y, ai = blockIdx.z;
+  const int ln = g.ln, ny = g.Ny, nx = g.Nx;
+  const cpx<R> *Tk = T + ((size_t)ai * ln + k) * g.Lx * ny;
+  for (int idx = threadIdx.x; idx < g.Lx * rb; idx += blockDim.x) {
+    const int cx = idx / rb, r = idx - cx * rb;
+    const int y = y0 + r;
+    tile[idx] = y < ny ? Tk[(size_t)cx * ny + y] : mk<R>(0, 0);
+  }
+  __syncthreads();
+  int cxm[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) cxm[m] = freq_of_bin<N>(j + m * TT) - g.jx0;
+  const int iters = (rb + tpb - 1) / tpb;
+  for (int it = 0; it < iters; ++it) {
+    const int r = it * tpb + t;
+    const int y = y0 + r;
+    const bool valid = r < rb && y < ny;
+    cpx<R> v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      v[m] = (valid && cxm[m] >= 0 && cxm[m] < g.Lx) ? tile[cxm[m] * rb + r]
+                                                     : mk<R>(0, 0);
+    FFT::run(v, b0, b1, twx, j);
+    cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
+    if (g.fuse_x)
+      boundary_terms<R, N, TT>(v, j, g.padxl, nx, (R)g.b1, corr, R0, R1);
+    if (valid) {
+      V *dst = slab + (((size_t)(slab0 + ai) * ny + y) * ln + k) * nx;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int x = j + m * TT - g.padxl;
+        if (x >= 0 && x < nx) {
+          cpx<R> val = v[m];
+          if (g.fuse_x) val = val + q0x[x] * R0 + q0x[nx - 1 - x] * R1;
+          store_val(dst + x, val);
+        }
+      }
+    }
+  }
+}
 
 // Row stage: rows y0..y0+nb-1 of slice k: inverse transform along x of the
 // support columns of T, crop columns padxl..padxl+Nx-1 with the fused
@@ -467,6 +628,79 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// In-place rotation with scipy's aliasing for spline orders 0 and 1.
+// The reference rotates each y-plane with
+// scipy.ndimage.rotate(arr[...], output=arr[...]) (_alg3d_bpp.py:73-81).
+// For order < 2 scipy does not prefilter, so it reads the array it is
+// writing: output points are produced in C order (z outer, x inner) and later
+// points see the already rotated values of earlier ones.  Every
+// (angle, y-plane, real/imag part) is one independent sequential job here,
+// replaying that order exactly on its own slab plane.
+// ---------------------------------------------------------------------------
+template <typename R, int ORDER>
+__global__ void k_seq_rotate(R *data, long long njobs, int ny, int nz, int nx,
+                             int ec, const double *__restrict__ params) {
+  const long long job = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (job >= njobs) return;
+  const long long a = job / ((long long)ny * ec);
+  const int rem = (int)(job - a * ny * ec);
+  const int y = rem / ec, comp = rem - y * ec;
+  R *pl = data + ((a * ny + y) * (long long)nz * nx) * ec + comp;
+  const double *p = params + 6 * a;
+  for (int z = 0; z < nz; ++z) {
+    for (int x = 0; x < nx; ++x) {
+      const double cz = affine2((double)z, (double)x, p[0], p[1], p[4]);
+      const double cx = affine2((double)z, (double)x, p[2], p[3], p[5]);
+      R v = R(0);
+      if (!(cz < 0.0 || cz > (double)(nz - 1) || cx < 0.0 ||
+            cx > (double)(nx - 1))) {
+        double wz[ORDER + 1], wx[ORDER + 1];
+        spline_weights<ORDER, double>(cz, wz);
+        spline_weights<ORDER, double>(cx, wx);
+        const int sz = spline_start<ORDER>(cz), sx = spline_start<ORDER>(cx);
+        double acc = 0.0;
+        for (int i = 0; i <= ORDER; ++i) {
+          const long long row = (long long)mirror_idx(sz + i, nz) * nx;
+          for (int j = 0; j <= ORDER; ++j)
+            acc += (double)pl[(row + mirror_idx(sx + j, nx)) * ec] * wz[i] * wx[j];
+        }
+        v = (R)acc;
+      }
+      pl[((long long)z * nx + x) * ec] = v;
+    }
+  }
+}
+
+// out[z][y][x] (+)= sum over the batch of slab[a][y][z][x], compensated.
+template <typename R, typename V>
+__global__ void __launch_bounds__(256)
+    k_sum_angles(const V *__restrict__ slab, int nb, V *__restrict__ out,
+                 V *__restrict__ comp, int ny, int nz, int nx, int first,
+                 int last) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long vol = (long long)ny * nz * nx;
+  if (v >= vol) return;
+  const int x = (int)(v % nx);
+  const long long zy = v / nx;
+  const int y = (int)(zy % ny), z = (int)(zy / ny);
+  V sum, cmp;
+  if (!first) {
+    sum = out[v];
+    if (comp)
+      cmp = comp[v];
+    else
+      acc_zero(cmp);
+  } else {
+    acc_zero(sum);
+    acc_zero(cmp);
+  }
+  const long long src = ((long long)y * nz + z) * nx + x;
+  for (int a = 0; a < nb; ++a) acc_add(sum, cmp, slab[a * vol + src]);
+  out[v] = sum;
+  if (!last && comp) comp[v] = cmp;
+}
+
+// ---------------------------------------------------------------------------
 // separable mirror-boundary B-spline prefilter, one thread per line
 // (Unser/Thevenaz recursive filter; exact mirror initialisation), computed
 // in float64.  Line l starts at (l / n0) * s1 + (l % n0) * s0, elements es.
@@ -523,6 +757,29 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// batched transform of contiguous vectors with the register radix-8 engine
+template <typename R, int N, int DIR>
+__global__ void __launch_bounds__(512)
+    k_fft8_batch(const cpx<R> *in, cpx<R> *out, long long batch,
+                 const cpx<R> *tw) {
+  using FFT = Fft8<R, N, DIR>;
+  constexpr int TT = FFT::T;
+  const int tpb = blockDim.x / TT;
+  const int t = threadIdx.x / TT, j = threadIdx.x - t * TT;
+  cpx<R> *lds = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *b0 = lds + 2 * t * FFT::PADN, *b1 = b0 + FFT::PADN;
+  const long long b = (long long)blockIdx.x * tpb + t;
+  cpx<R> v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    v[m] = b < batch ? in[b * N + j + m * TT] : mk<R>(0, 0);
+  FFT::run(v, b0, b1, tw, j);
+  if (b < batch) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) out[b * N + j + m * TT] = v[m];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -567,11 +824,78 @@ hipError_t launch_fwd_cols(const cpx<R> *F1, cpx<R> *S, const cpx<R> *twy,
   return hipGetLastError();
 }
 
+static constexpr size_t kLdsMax = 160 * 1024;
+
+template <typename R, int N>
+static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
+                                  const cpx<R> *twy, const double *pa,
+                                  const PropAux<R> *aux, const R *q0z,
+                                  const R *q0y, const Geom &g, int na,
+                                  hipStream_t st) {
+  constexpr int TT = N / 8;
+  const int threads = TT < 256 ? 256 : TT;
+  const int tpb = threads / TT;
+  const size_t sh = (2 * (size_t)tpb * lds_padded(N) + 4 * tpb) * sizeof(cpx<R>);
+  if (sh > kLdsMax) return hipErrorInvalidValue;
+  hipError_t e = set_lds((const void *)k_col_fft8<R, N>, sh);
+  if (e != hipSuccess) return e;
+  const int kb = 16;
+  dim3 grid(g.Lx, (g.ln + kb - 1) / kb, na);
+  k_col_fft8<R, N><<<grid, threads, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g, kb);
+  return hipGetLastError();
+}
+
+template <typename R, typename V, int N>
+static hipError_t row_fft8_launch(const cpx<R> *T, V *slab, const cpx<R> *twx,
+                                  const R *q0x, const Geom &g, int na,
+                                  int slab0, hipStream_t st) {
+  constexpr int TT = N / 8;
+  const int threads = TT < 256 ? 256 : TT;
+  const int tpb = threads / TT;
+  const size_t fft = (2 * (size_t)tpb * lds_padded(N) + 4 * tpb) * sizeof(cpx<R>);
+  if (fft >= kLdsMax) return hipErrorInvalidValue;
+  int rb = (int)std::min<size_t>(16, (kLdsMax - fft) / ((size_t)g.Lx * sizeof(cpx<R>)));
+  if (rb > g.Ny) rb = g.Ny;
+  if (rb < 1) return hipErrorInvalidValue;
+  const size_t sh = fft + (size_t)g.Lx * rb * sizeof(cpx<R>);
+  hipError_t e = set_lds((const void *)k_row_fft8<R, V, N>, sh);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.Ny + rb - 1) / rb, g.ln, na);
+  k_row_fft8<R, V, N><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g, rb, slab0);
+  return hipGetLastError();
+}
+
+// fast-path eligibility: power of two 64..4096 and the LDS footprint fits
+template <typename R>
+static bool fft8_col_ok(const Geom &g) {
+  if (g.logNy < 6 || g.logNy > 12) return false;
+  const int TT = g.lNy / 8, tpb = TT < 256 ? 256 / TT : 1;
+  return (2 * (size_t)tpb * lds_padded(g.lNy) + 4 * tpb) * sizeof(cpx<R>) <= kLdsMax;
+}
+template <typename R>
+static bool fft8_row_ok(const Geom &g) {
+  if (g.logNx < 6 || g.logNx > 12) return false;
+  const int TT = g.lNx / 8, tpb = TT < 256 ? 256 / TT : 1;
+  const size_t fft = (2 * (size_t)tpb * lds_padded(g.lNx) + 4 * tpb) * sizeof(cpx<R>);
+  return fft + (size_t)g.Lx * sizeof(cpx<R>) <= kLdsMax;
+}
+
 template <typename R>
 hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
                            const double *pa, const PropAux<R> *aux,
                            const R *q0z, const R *q0y, const Geom &g, int na,
                            int nb, hipStream_t st) {
+  if (fft8_col_ok<R>(g)) {
+    switch (g.lNy) {
+      case 64: return col_fft8_launch<R, 64>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+      case 128: return col_fft8_launch<R, 128>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+      case 256: return col_fft8_launch<R, 256>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+      case 512: return col_fft8_launch<R, 512>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+      case 1024: return col_fft8_launch<R, 1024>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+      case 2048: return col_fft8_launch<R, 2048>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+      case 4096: return col_fft8_launch<R, 4096>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+    }
+  }
   const size_t sh = lds_bytes(g.lNy, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_col_ifft<R>, sh);
   if (e != hipSuccess) return e;
@@ -584,6 +908,17 @@ template <typename R, typename V>
 hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
                            const R *q0x, const Geom &g, int na, int nb,
                            int slab0, hipStream_t st) {
+  if (fft8_row_ok<R>(g)) {
+    switch (g.lNx) {
+      case 64: return row_fft8_launch<R, V, 64>(T, slab, twx, q0x, g, na, slab0, st);
+      case 128: return row_fft8_launch<R, V, 128>(T, slab, twx, q0x, g, na, slab0, st);
+      case 256: return row_fft8_launch<R, V, 256>(T, slab, twx, q0x, g, na, slab0, st);
+      case 512: return row_fft8_launch<R, V, 512>(T, slab, twx, q0x, g, na, slab0, st);
+      case 1024: return row_fft8_launch<R, V, 1024>(T, slab, twx, q0x, g, na, slab0, st);
+      case 2048: return row_fft8_launch<R, V, 2048>(T, slab, twx, q0x, g, na, slab0, st);
+      case 4096: return row_fft8_launch<R, V, 4096>(T, slab, twx, q0x, g, na, slab0, st);
+    }
+  }
   const size_t sh = lds_bytes(g.lNx, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_row_ifft<R, V>, sh);
   if (e != hipSuccess) return e;
@@ -652,6 +987,30 @@ hipError_t launch_affine_acc(int order, const V *coef, int nb,
   return hipErrorInvalidValue;
 }
 
+template <typename R, typename V>
+hipError_t launch_seq_rotate_sum(int order, V *slab, int nb,
+                                 const double *params, V *out, V *comp,
+                                 int ny, int nz, int nx, int first, int last,
+                                 hipStream_t st) {
+  const int ec = sizeof(V) / sizeof(R);
+  const long long njobs = (long long)nb * ny * ec;
+  const unsigned blocks = (unsigned)((njobs + 63) / 64);
+  if (order == 0)
+    k_seq_rotate<R, 0><<<blocks, 64, 0, st>>>((R *)slab, njobs, ny, nz, nx, ec,
+                                              params);
+  else if (order == 1)
+    k_seq_rotate<R, 1><<<blocks, 64, 0, st>>>((R *)slab, njobs, ny, nz, nx, ec,
+                                              params);
+  else
+    return hipErrorInvalidValue;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long vol = (long long)ny * nz * nx;
+  k_sum_angles<R, V><<<(unsigned)((vol + 255) / 256), 256, 0, st>>>(
+      slab, nb, out, comp, ny, nz, nx, first, last);
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_iir(R *data, long long nlines, int n, long long n0,
                       long long s0, long long s1, long long es, int order,
@@ -682,10 +1041,47 @@ hipError_t launch_iir(R *data, long long nlines, int n, long long n0,
   return hipGetLastError();
 }
 
+template <typename R, int N>
+static hipError_t fft8_batch_launch(const cpx<R> *in, cpx<R> *out,
+                                    long long batch, int dir, const cpx<R> *tw,
+                                    hipStream_t st) {
+  constexpr int TT = N / 8;
+  const int threads = TT < 256 ? 256 : TT;
+  const int tpb = threads / TT;
+  const size_t sh = 2 * (size_t)tpb * lds_padded(N) * sizeof(cpx<R>);
+  if (sh > kLdsMax) return hipErrorInvalidValue;
+  const void *fn = dir < 0 ? (const void *)k_fft8_batch<R, N, -1>
+                           : (const void *)k_fft8_batch<R, N, +1>;
+  hipError_t e = set_lds(fn, sh);
+  if (e != hipSuccess) return e;
+  const unsigned blocks = (unsigned)((batch + tpb - 1) / tpb);
+  if (dir < 0)
+    k_fft8_batch<R, N, -1><<<blocks, threads, sh, st>>>(in, out, batch, tw);
+  else
+    k_fft8_batch<R, N, +1><<<blocks, threads, sh, st>>>(in, out, batch, tw);
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
                             int n, int logn, int dir, const cpx<R> *tw, int nb,
                             hipStream_t st) {
+  if (logn >= 3 && logn <= 12 &&
+      2 * (size_t)(n / 8 < 256 ? 256 / (n / 8) : 1) * lds_padded(n) *
+              sizeof(cpx<R>) <= kLdsMax) {
+    switch (n) {
+      case 8: return fft8_batch_launch<R, 8>(in, out, batch, dir, tw, st);
+      case 16: return fft8_batch_launch<R, 16>(in, out, batch, dir, tw, st);
+      case 32: return fft8_batch_launch<R, 32>(in, out, batch, dir, tw, st);
+      case 64: return fft8_batch_launch<R, 64>(in, out, batch, dir, tw, st);
+      case 128: return fft8_batch_launch<R, 128>(in, out, batch, dir, tw, st);
+      case 256: return fft8_batch_launch<R, 256>(in, out, batch, dir, tw, st);
+      case 512: return fft8_batch_launch<R, 512>(in, out, batch, dir, tw, st);
+      case 1024: return fft8_batch_launch<R, 1024>(in, out, batch, dir, tw, st);
+      case 2048: return fft8_batch_launch<R, 2048>(in, out, batch, dir, tw, st);
+      case 4096: return fft8_batch_launch<R, 4096>(in, out, batch, dir, tw, st);
+    }
+  }
   const size_t sh = lds_bytes(n, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_fft_batch<R>, sh);
   if (e != hipSuccess) return e;
@@ -730,6 +1126,13 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
   template hipError_t launch_affine_acc<R, cpx<R>>(                            \
       int, const cpx<R> *, int, const double *, cpx<R> *, cpx<R> *, int, int,  \
       int, int, int, hipStream_t);                                             \
+  template hipError_t launch_seq_rotate_sum<R, R>(int, R *, int,              \
+                                                  const double *, R *, R *,    \
+                                                  int, int, int, int, int,     \
+                                                  hipStream_t);                \
+  template hipError_t launch_seq_rotate_sum<R, cpx<R>>(                        \
+      int, cpx<R> *, int, const double *, cpx<R> *, cpx<R> *, int, int, int,   \
+      int, int, hipStream_t);                                                  \
   template hipError_t launch_iir<R>(R *, long long, int, long long, long long, \
                                     long long, long long, int, hipStream_t);   \
   template hipError_t launch_fft_batch<R>(const cpx<R> *, cpx<R> *, long long, \

@@ -59,6 +59,11 @@ hipError_t launch_affine_acc(int order, const V *coef, int nb,
                              const double *params, V *out, V *comp, int ny,
                              int nz, int nx, int first, int last,
                              hipStream_t st);
+template <typename R, typename V>
+hipError_t launch_seq_rotate_sum(int order, V *slab, int nb,
+                                 const double *params, V *out, V *comp,
+                                 int ny, int nz, int nx, int first, int last,
+                                 hipStream_t st);
 template <typename R>
 hipError_t launch_iir(R *data, long long nlines, int n, long long n0,
                       long long s0, long long s1, long long es, int order,

@@ -296,6 +296,10 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
       HIPCHK((odt::launch_affine_acc<R, V>(p->order, slab, nb, prm, out, comp, g.Ny,
                                            g.ln, g.Nx, first, last, st)),
              "k_affine_acc");
+    else if (p->kind == ODT_KIND_3D && p->order < 2)
+      HIPCHK((odt::launch_seq_rotate_sum<R, V>(p->order, slab, nb, prm, out, comp,
+                                               g.Ny, g.ln, g.Nx, first, last, st)),
+             "k_seq_rotate");
     else
       HIPCHK((odt::launch_rotate_acc<R, V>(p->order, slab, nb, prm, out, comp, g.Ny,
                                            g.ln, g.Nx, first, last, st)),

@@ -57,16 +57,20 @@ def test_2d_fixtures(name, ampl):
 
 
 def test_3d_fp32_against_fp64_fixture_inputs():
-    """The reference's FP32 bar (atol 6e-7) and the project's rel-L2 bar."""
+    """The project's rel-L2 bar and the reference's FP32 bar (atol 6e-7 on
+    the cut-out volume, tests/test_alg3d_bpp.py:101-113)."""
     sino, angles = create_test_sino_3d()
     for p in get_test_parameter_set(2):
         f32 = G.backpropagate_3d(sino, angles, padval=0, dtype=np.float32,
                                  **p)
         ref = O.backpropagate_3d(sino, angles, padval=0, dtype=float, **p)
         assert f32.dtype == np.complex64
-        assert rel_l2(f32, ref) < FP32_REL_L2
-        assert np.allclose(cutout(f32.copy()), cutout(ref.copy()), atol=6e-7,
-                           rtol=0)
+        err = rel_l2(f32, ref)
+        d = cutout(f32.astype(np.complex128)) - cutout(ref.copy())
+        maxabs = float(np.abs(d.view(float)).max())
+        print(f"fp32 fixture inputs {p}: rel_l2={err:.2e} maxabs={maxabs:.2e}")
+        assert err < FP32_REL_L2
+        assert maxabs < 6e-7, maxabs
 
 
 @pytest.mark.parametrize("order", [0, 1, 2, 3, 4, 5])
