@@ -220,13 +220,4 @@ __device__ __forceinline__ double affine3(double o0, double o1, double o2,
   return t + off;
 }
 
-// Compensated (Kahan) accumulation.
-template <typename R>
-__device__ __forceinline__ void kahan_add(R &sum, R &comp, R v) {
-  const R y = v - comp;
-  const R t = sum + y;
-  comp = (t - sum) - y;
-  sum = t;
-}
-
 }  // namespace odt

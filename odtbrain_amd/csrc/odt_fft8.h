@@ -8,8 +8,12 @@
 // y[(b - k) R + k + q L].  Thread j owns butterflies b = j + i*T.
 // Input and output use the same "strided" layout: register m of thread j is
 // element j + m*T, so global loads and stores are coalesced across threads.
-// Between stages the data goes through LDS (ping-pong buffers, one barrier
-// per exchange, one padding slot per 16 elements against bank conflicts).
+// Between stages the data goes through LDS with one padding slot per 16
+// elements against bank conflicts.  When a transform fits in one wavefront
+// (N <= 512) the exchange needs no barrier: a wave's LDS operations execute
+// in program order, so each stage reads and then overwrites one private
+// buffer.  Larger transforms span several waves and use ping-pong buffers
+// with one __syncthreads() per exchange.
 #pragma once
 
 #include "odt_device.h"
@@ -104,14 +108,17 @@ struct Fft8 {
     constexpr int NB = 8 / R;
     constexpr bool FIRST = S == 0;
     constexpr bool LAST = S == NS - 1;
-    cpx<F> *src = (S & 1) ? buf0 : buf1;  // written by stage S-1
-    cpx<F> *dst = (S & 1) ? buf1 : buf0;
+    constexpr bool WAVE = T <= 64;
+    cpx<F> *src = WAVE ? buf0 : ((S & 1) ? buf0 : buf1);  // from stage S-1
+    cpx<F> *dst = WAVE ? buf0 : ((S & 1) ? buf1 : buf0);
     cpx<F> a[8];
 #pragma unroll
     for (int i = 0; i < NB; ++i)
 #pragma unroll
       for (int r = 0; r < R; ++r)
         a[i * R + r] = FIRST ? v[i + r * NB] : src[lds_pad(j + i * T + r * (N / R))];
+    // wave mode: no compiler motion of this stage's LDS writes above its reads
+    if (WAVE && !FIRST) __atomic_signal_fence(__ATOMIC_SEQ_CST);
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const int b = j + i * T;
@@ -139,7 +146,10 @@ struct Fft8 {
         for (int q = 0; q < R; ++q)
           dst[lds_pad((b - k) * R + k + q * L)] = a[i * R + q];
       }
-      __syncthreads();
+      if (WAVE)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      else
+        __syncthreads();
     }
   }
 
@@ -151,14 +161,17 @@ struct Fft8 {
     if constexpr (S + 1 < NS) stages<S + 1, L * radix(S)>(v, b0, b1, tw, j);
   }
 
-  // v[m] = x[j + m*T] on entry, X[j + m*T] on exit.  buf0/buf1: this
-  // transform's two LDS scratch buffers of PADN elements.  Must be called
-  // by all threads of the block (it contains barriers).
+  // Buffers one transform needs (PADN elements each).
+  static constexpr int NBUF = T <= 64 ? 1 : 2;
+
+  // v[m] = x[j + m*T] on entry, X[j + m*T] on exit.  buf0 (and buf1 when
+  // NBUF == 2): this transform's LDS scratch.  For NBUF == 2 all threads of
+  // the block must call it (it contains barriers).
   __device__ static __forceinline__ void run(cpx<F> *v, cpx<F> *buf0,
                                              cpx<F> *buf1, const cpx<F> *tw,
                                              int j) {
     // the previous call's last stage may still be reading the buffers
-    if (NS > 1) __syncthreads();
+    if (NBUF == 2 && NS > 1) __syncthreads();
     stages<0, 1>(v, buf0, buf1, tw, j);
   }
 };

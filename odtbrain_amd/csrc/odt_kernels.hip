@@ -13,7 +13,7 @@
 //                the fused z-prefilter), inverse transform along y, crop (T)
 //   k_row_ifft   inverse transform along x, crop, fused x-prefilter; writes
 //                the B-spline coefficient slab of the projection
-//   k_rotate_acc / k_affine_acc
+//   k_rotate_lds / k_affine_acc
 //                resample every slab of the batch at the rotated position
 //                of each output voxel and sum over angles (compensated)
 //
@@ -168,10 +168,12 @@ __global__ void __launch_bounds__(256)
 // modulo 2*pi before the single-precision sincos
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ cpx<float> phasor(double phi, float) {
-  const double r = phi - 6.283185307179586 * rint(phi * 0.15915494309189535);
-  float s, c;
-  sincosf((float)r, &s, &c);
-  return mk<float>(c, s);
+  // phase in revolutions, reduced to [-0.5, 0.5] in float64; the hardware
+  // sin/cos take revolutions (max abs error 1.3e-7 on this range, measured
+  // against float64 on gfx950 -- no worse than sincosf's 1.5e-7)
+  const double rev = phi * 0.15915494309189535;
+  const float r = (float)(rev - rint(rev));
+  return mk<float>(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
 }
 __device__ __forceinline__ cpx<double> phasor(double phi, double) {
   double s, c;
@@ -182,7 +184,7 @@ __device__ __forceinline__ cpx<double> phasor(double phi, double) {
 // Column stage: for x-frequency column c and slices k0..k0+nb-1 build
 // S*g_k along the y-frequencies (zero outside the support), inverse
 // transform along y, crop rows padyl..padyl+Ny-1 (plus the fused
-// y-prefilter boundary terms for the tilted path), write T[a][k][c][y].
+// y-prefilter boundary terms for the tilted path), write T[a][k][y][c].
 template <typename R>
 __global__ void __launch_bounds__(256)
     k_col_ifft(const cpx<R> *__restrict__ S, cpx<R> *__restrict__ T,
@@ -190,7 +192,7 @@ __global__ void __launch_bounds__(256)
                const double *__restrict__ prop_a,
                const PropAux<R> *__restrict__ aux,
                const R *__restrict__ q0z, const R *__restrict__ q0y, Geom g,
-               int nb) {
+               int nb, int ldt) {
   cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
   cpx<R> *tmp = buf + nb * g.lNy;
   __shared__ cpx<R> corr[2 * 16];
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(256)
     cpx<R> v = res[r * g.lNy + wrap(g.padyl + y, g.lNy)];
     if (g.fuse_y)
       v = v + q0y[y] * corr[2 * r] + q0y[ny - 1 - y] * corr[2 * r + 1];
-    T[(((size_t)ai * ln + k) * g.Lx + c) * ny + y] = v;
+    T[(((size_t)ai * ln + k) * ny + y) * ldt + c] = v;
   }
 }
 
@@ -255,12 +257,14 @@ __device__ __forceinline__ int freq_of_bin(int bin) {
 // outputs Y[p-1], Y[p+1], Y[p+n-2], Y[p+n] (indices mod N, p = crop start,
 // n = crop length) publish them in `corr`; returns
 // R0 = b1 (Y[p-1] - Y[p+1]) and R1 = b1 (Y[p+n] - Y[p+n-2]).
-template <typename R, int N, int TT>
+// WAVE: the transform lives in one wavefront (no block barrier needed).
+template <typename R, int N, int TT, bool WAVE>
 __device__ __forceinline__ void boundary_terms(const cpx<R> *v, int j, int p,
                                                int n, R b1, cpx<R> *corr,
                                                cpx<R> &R0, cpx<R> &R1) {
   const int sb[4] = {(p - 1 + N) & (N - 1), (p + 1) & (N - 1),
                      (p + n - 2 + N) & (N - 1), (p + n) & (N - 1)};
+  if (!WAVE) __syncthreads();
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const int bin = j + m * TT;
@@ -268,13 +272,25 @@ __device__ __forceinline__ void boundary_terms(const cpx<R> *v, int j, int p,
     for (int i = 0; i < 4; ++i)
       if (bin == sb[i]) corr[i] = v[m];
   }
-  __syncthreads();
+  if (WAVE)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  else
+    __syncthreads();
   R0 = b1 * (corr[0] - corr[1]);
   R1 = b1 * (corr[3] - corr[2]);
+  if (WAVE) __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // Column stage, register radix-8 engine (power-of-two lNy = N).
-// Block: tpb = blockDim/(N/8) transforms; column c, slices k0..k0+kb-1.
+// A block owns CPB = 8 adjacent support columns c0..c0+7 and the slices
+// k0..k0+kb-1; its tpb transform slots take the columns tpb at a time.  The
+// S column and the tables are read from cache per use (keeping them out of
+// the transform's registers).  Outputs are transposed through an LDS tile
+// so that each row of T[a][k][y][c] (stride ldt) is written in contiguous
+// 8-column segments; without room for the tile (very long columns) the
+// transform writes T directly.
+constexpr int kColCpb = 8;
+
 template <typename R, int N>
 __global__ void __launch_bounds__(512)
     k_col_fft8(const cpx<R> *__restrict__ S, cpx<R> *__restrict__ T,
@@ -282,95 +298,97 @@ __global__ void __launch_bounds__(512)
                const double *__restrict__ prop_a,
                const PropAux<R> *__restrict__ aux,
                const R *__restrict__ q0z, const R *__restrict__ q0y, Geom g,
-               int kb) {
+               int kb, int ldt, int staged) {
   using FFT = Fft8<R, N, +1>;
   constexpr int TT = FFT::T;
+  constexpr bool WAVE = TT <= 64;
   const int tpb = blockDim.x / TT;
   const int t = threadIdx.x / TT, j = threadIdx.x - t * TT;
   cpx<R> *lds = reinterpret_cast<cpx<R> *>(odt_smem);
-  cpx<R> *b0 = lds + 2 * t * FFT::PADN, *b1 = b0 + FFT::PADN;
-  cpx<R> *corr = lds + 2 * tpb * FFT::PADN + 4 * t;
-  const int c = blockIdx.x, k0 = blockIdx.y * kb, ai = blockIdx.z;
+  cpx<R> *b0 = lds + FFT::NBUF * t * FFT::PADN, *b1 = b0 + FFT::PADN;
+  cpx<R> *corr = lds + FFT::NBUF * tpb * FFT::PADN + 4 * t;
+  cpx<R> *tile = lds + FFT::NBUF * tpb * FFT::PADN + 4 * tpb;  // ny x (CPB+1)
+  const int c0 = blockIdx.x * kColCpb;
+  const int k0 = blockIdx.y * kb, ai = blockIdx.z;
   const int ln = g.ln, ny = g.Ny;
   const int kend = min(k0 + kb, ln);
-  cpx<R> sv[8];
-  double pa[8];
-  PropAux<R> ax[8];
-  unsigned inside = 0;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const int cy = freq_of_bin<N>(j + m * TT) - g.jy0;
-    sv[m] = mk<R>(0, 0);
-    pa[m] = 0.0;
-    if (cy >= 0 && cy < g.Ly) {
-      inside |= 1u << m;
-      const size_t tix = (size_t)c * g.Ly + cy;
-      sv[m] = S[((size_t)ai * g.Lx + c) * g.Ly + cy];
-      pa[m] = prop_a[tix];
-      if (g.fuse_z) ax[m] = aux[tix];
-    }
-  }
-  const int iters = (kb + tpb - 1) / tpb;
-  for (int it = 0; it < iters; ++it) {
-    const int k = k0 + it * tpb + t;
-    const bool valid = k < kend;
+  for (int k = k0; k < kend; ++k) {
     const double z = g.z0 + (double)k;
-    cpx<R> v[8];
+    // z-prefilter boundary terms only where q0 is not negligible
+    const R qa = g.fuse_z ? q0z[k] : R(0);
+    const R qb = g.fuse_z ? q0z[ln - 1 - k] : R(0);
+    const bool edge = qa != R(0) || qb != R(0);
+    cpx<R> *trow = T + (((size_t)ai * ln + k) * ny) * ldt;
+    for (int cb = 0; cb < kColCpb; cb += tpb) {
+      const int cl = cb + t;  // local column of this slot
+      const int c = c0 + cl;
+      const bool live = cl < kColCpb && c < g.Lx;
+      const cpx<R> *Sc = S + ((size_t)ai * g.Lx + c) * g.Ly;
+      const size_t tr = (size_t)c * g.Ly;
+      cpx<R> v[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      v[m] = mk<R>(0, 0);
-      if (valid && ((inside >> m) & 1u)) {
-        const cpx<R> e = phasor(pa[m] * z, R(0));
-        cpx<R> gk = e;
-        if (g.fuse_z)
-          gk = ax[m].ib * e + q0z[k] * ax[m].r0 + q0z[ln - 1 - k] * ax[m].r1;
-        v[m] = sv[m] * gk;
+      for (int m = 0; m < 8; ++m) {
+        v[m] = mk<R>(0, 0);
+        const int cy = freq_of_bin<N>(j + m * TT) - g.jy0;
+        if (live && cy >= 0 && cy < g.Ly) {
+          cpx<R> gk = phasor(prop_a[tr + cy] * z, R(0));
+          if (g.fuse_z) {
+            const PropAux<R> *x = aux + tr + cy;
+            gk = x->ib * gk;
+            if (edge) gk = gk + qa * x->r0 + qb * x->r1;
+          }
+          v[m] = Sc[cy] * gk;
+        }
       }
-    }
-    FFT::run(v, b0, b1, twy, j);
-    cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
-    if (g.fuse_y)
-      boundary_terms<R, N, TT>(v, j, g.padyl, ny, (R)g.b1, corr, R0, R1);
-    if (valid) {
-      cpx<R> *dst = T + (((size_t)ai * ln + k) * g.Lx + c) * ny;
+      FFT::run(v, b0, b1, twy, j);
+      cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
+      if (g.fuse_y)
+        boundary_terms<R, N, TT, WAVE>(v, j, g.padyl, ny, (R)g.b1, corr, R0,
+                                       R1);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int y = j + m * TT - g.padyl;
-        if (y >= 0 && y < ny) {
+        if (live && y >= 0 && y < ny) {
           cpx<R> val = v[m];
           if (g.fuse_y) val = val + q0y[y] * R0 + q0y[ny - 1 - y] * R1;
-          dst[y] = val;
+          if (staged)
+            tile[y * (kColCpb + 1) + cl] = val;
+          else
+            trow[(size_t)y * ldt + c] = val;
         }
       }
+    }
+    if (staged) {
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < ny * kColCpb; idx += blockDim.x) {
+        const int y = idx / kColCpb, cl = idx - y * kColCpb;
+        if (c0 + cl < g.Lx)
+          trow[(size_t)y * ldt + c0 + cl] = tile[y * (kColCpb + 1) + cl];
+      }
+      __syncthreads();
     }
   }
 }
 
-// Row stage, register radix-8 engine (power-of-two lNx = N).  The block
-// stages the support columns of rows y0..y0+rb-1 of slice k in LDS
-// (coalesced reads of T[a][k][c][y]), then transforms them tpb at a time.
+// Row stage, register radix-8 engine (power-of-two lNx = N).  Each
+// transform reads one row T[a][k][y][0..Lx) (contiguous, coalesced),
+// transforms it, crops with the fused x-prefilter terms and writes one slab
+// row.  Block: rows y0..y0+rb-1 of slice k, tpb rows at a time.
 template <typename R, typename V, int N>
 __global__ void __launch_bounds__(512)
     k_row_fft8(const cpx<R> *__restrict__ T, V *__restrict__ slab,
                const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
-               Geom g, int rb, int slab0) {
+               Geom g, int rb, int slab0, int ldt) {
   using FFT = Fft8<R, N, +1>;
   constexpr int TT = FFT::T;
+  constexpr bool WAVE = TT <= 64;
   const int tpb = blockDim.x / TT;
   const int t = threadIdx.x / TT, j = threadIdx.x - t * TT;
   cpx<R> *lds = reinterpret_cast<cpx<R> *>(odt_smem);
-  cpx<R> *b0 = lds + 2 * t * FFT::PADN, *b1 = b0 + FFT::PADN;
-  cpx<R> *corr = lds + 2 * tpb * FFT::PADN + 4 * t;
-  cpx<R> *tile = lds + 2 * tpb * FFT::PADN + 4 * tpb;
+  cpx<R> *b0 = lds + FFT::NBUF * t * FFT::PADN, *b1 = b0 + FFT::PADN;
+  cpx<R> *corr = lds + FFT::NBUF * tpb * FFT::PADN + 4 * t;
   const int y0 = blockIdx.x * rb, k = blockIdx.y, ai = blockIdx.z;
   const int ln = g.ln, ny = g.Ny, nx = g.Nx;
-  const cpx<R> *Tk = T + ((size_t)ai * ln + k) * g.Lx * ny;
-  for (int idx = threadIdx.x; idx < g.Lx * rb; idx += blockDim.x) {
-    const int cx = idx / rb, r = idx - cx * rb;
-    const int y = y0 + r;
-    tile[idx] = y < ny ? Tk[(size_t)cx * ny + y] : mk<R>(0, 0);
-  }
-  __syncthreads();
   int cxm[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) cxm[m] = freq_of_bin<N>(j + m * TT) - g.jx0;
@@ -379,15 +397,16 @@ __global__ void __launch_bounds__(512)
     const int r = it * tpb + t;
     const int y = y0 + r;
     const bool valid = r < rb && y < ny;
+    if (WAVE && !valid) break;  // rows are taken in order: nothing left
+    const cpx<R> *src = T + (((size_t)ai * ln + k) * ny + y) * ldt;
     cpx<R> v[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m)
-      v[m] = (valid && cxm[m] >= 0 && cxm[m] < g.Lx) ? tile[cxm[m] * rb + r]
-                                                     : mk<R>(0, 0);
+      v[m] = (valid && cxm[m] >= 0 && cxm[m] < g.Lx) ? src[cxm[m]] : mk<R>(0, 0);
     FFT::run(v, b0, b1, twx, j);
     cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
     if (g.fuse_x)
-      boundary_terms<R, N, TT>(v, j, g.padxl, nx, (R)g.b1, corr, R0, R1);
+      boundary_terms<R, N, TT, WAVE>(v, j, g.padxl, nx, (R)g.b1, corr, R0, R1);
     if (valid) {
       V *dst = slab + (((size_t)(slab0 + ai) * ny + y) * ln + k) * nx;
 #pragma unroll
@@ -403,14 +422,15 @@ __global__ void __launch_bounds__(512)
   }
 }
 
-// Row stage: rows y0..y0+nb-1 of slice k: inverse transform along x of the
-// support columns of T, crop columns padxl..padxl+Nx-1 with the fused
-// x-prefilter boundary terms, write the coefficient slab [s][y][k][x].
+// Row stage (generic transform length): rows y0..y0+nb-1 of slice k:
+// inverse transform along x of the support columns of T, crop columns
+// padxl..padxl+Nx-1 with the fused x-prefilter boundary terms, write the
+// coefficient slab [s][y][k][x].
 template <typename R, typename V>
 __global__ void __launch_bounds__(256)
     k_row_ifft(const cpx<R> *__restrict__ T, V *__restrict__ slab,
                const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
-               Geom g, int nb, int slab0) {
+               Geom g, int nb, int slab0, int ldt) {
   cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
   cpx<R> *tmp = buf + nb * g.lNx;
   __shared__ cpx<R> corr[2 * 16];
@@ -421,11 +441,11 @@ __global__ void __launch_bounds__(256)
   for (int idx = threadIdx.x; idx < nb * g.lNx; idx += blockDim.x)
     buf[idx] = mk<R>(0, 0);
   __syncthreads();
-  const cpx<R> *Tk = T + ((size_t)ai * ln + k) * g.Lx * ny;
+  const cpx<R> *Tk = T + ((size_t)ai * ln + k) * ny * ldt;
   for (int idx = threadIdx.x; idx < g.Lx * nb; idx += blockDim.x) {
-    const int c = idx / nb, r = idx - c * nb;
+    const int r = idx / g.Lx, c = idx - r * g.Lx;
     const int y = y0 + r;
-    if (y < ny) buf[r * g.lNx + wrap(g.jx0 + c, g.lNx)] = Tk[(size_t)c * ny + y];
+    if (y < ny) buf[r * g.lNx + wrap(g.jx0 + c, g.lNx)] = Tk[(size_t)y * ldt + c];
   }
   __syncthreads();
   const cpx<R> *res = lds_transform<R>(buf, tmp, g.lNx, g.logNx, nb, twx, +1);
@@ -453,24 +473,56 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// rotate-accumulate about the y axis (scipy.ndimage.rotate(axes=(0,2)) of
-// every y plane, _alg3d_bpp.py:68-81, summed over the angles of the batch)
+// accumulation helpers of the resampling kernels (scipy.ndimage.rotate /
+// affine_transform of every slab, summed over the angles of the batch,
+// _alg3d_bpp.py:68-81 and 541-546, _alg3d_bppt.py:931-944)
 // ---------------------------------------------------------------------------
-template <typename R>
-struct Acc {
-  R s, c;
-};
 template <typename R>
 __device__ __forceinline__ void acc_zero(R &v) { v = R(0); }
 template <typename R>
 __device__ __forceinline__ void acc_zero(cpx<R> &v) { v = mk<R>(0, 0); }
-template <typename R>
-__device__ __forceinline__ void acc_add(R &s, R &c, R v) { kahan_add(s, c, v); }
-template <typename R>
-__device__ __forceinline__ void acc_add(cpx<R> &s, cpx<R> &c, cpx<R> v) {
-  kahan_add(s.re, c.re, v.re);
-  kahan_add(s.im, c.im, v.im);
+// float64 running sums for the angle accumulation: wide_t<V> holds V in
+// float64; the float32 compensation kept in HBM between launches is
+// (rounded - exact), as in Kahan's scheme.
+template <typename V> struct wide_of { using type = double; };
+template <> struct wide_of<cpx<float>> { using type = cpx<double>; };
+template <> struct wide_of<cpx<double>> { using type = cpx<double>; };
+template <typename V> using wide_t = typename wide_of<V>::type;
+__device__ __forceinline__ double widen(float v) { return (double)v; }
+__device__ __forceinline__ double widen(double v) { return v; }
+__device__ __forceinline__ cpx<double> widen(cpx<float> v) {
+  return mk<double>((double)v.re, (double)v.im);
 }
+__device__ __forceinline__ cpx<double> widen(cpx<double> v) { return v; }
+template <typename V>
+__device__ __forceinline__ V narrow(double v) { return (V)v; }
+template <typename V>
+__device__ __forceinline__ V narrow(cpx<double> v);
+template <>
+__device__ __forceinline__ cpx<float> narrow<cpx<float>>(cpx<double> v) {
+  return mk<float>((float)v.re, (float)v.im);
+}
+template <>
+__device__ __forceinline__ cpx<double> narrow<cpx<double>>(cpx<double> v) {
+  return v;
+}
+// load the running sum: exact ~= out - comp
+template <typename V>
+__device__ __forceinline__ wide_t<V> load_sum(const V *out, const V *comp,
+                                              size_t o) {
+  wide_t<V> s = widen(out[o]);
+  if (comp) s = s - widen(comp[o]);
+  return s;
+}
+// store the running sum (and its compensation if another launch follows)
+template <typename V>
+__device__ __forceinline__ void store_sum(V *out, V *comp, size_t o,
+                                          wide_t<V> s, bool more) {
+  const V r = narrow<V>(s);
+  out[o] = r;
+  if (more && comp) comp[o] = narrow<V>(widen(r) - s);
+}
+
 template <typename R>
 __device__ __forceinline__ R fma_w(R w, R v, R acc) { return acc + w * v; }
 template <typename R>
@@ -478,76 +530,241 @@ __device__ __forceinline__ cpx<R> fma_w(R w, cpx<R> v, cpx<R> acc) {
   return mk<R>(acc.re + w * v.re, acc.im + w * v.im);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged rotate-accumulate.  A block owns a 16x16 (z,x) output tile and
+// YC consecutive y planes.  Per angle it loads the input footprint of the
+// tile (bounding box of the rotated tile plus the spline support, mirror
+// extension applied while loading) for its YC planes into LDS with direct
+// global->LDS loads (global_load_lds: one wave instruction fills one 256-byte
+// tile row, no VGPR round trip), then every thread gathers its taps from
+// LDS.  The
+// input coordinates are computed exactly as scipy does (float64, no FMA), so
+// the out-of-bounds set is the reference's.  Sums over angles run in
+// float64 registers (float64 adds cost the same as float32 ones on this
+// GPU); between launches the float32 total carries a compensation term.
+// Blocks are numbered so that all tiles of one y-chunk land on the same XCD
+// (hardware dispatch is round-robin over the 8 XCDs): they share the
+// chunk's planes in that XCD's L2.
+// ---------------------------------------------------------------------------
+constexpr int kRotTile = 16;   // output tile edge (z and x)
+constexpr int kRotFoot = 32;   // LDS footprint edge per plane (>= 15*sqrt2+8)
+
+typedef __attribute__((address_space(3))) void *lds_vptr;
+typedef __attribute__((address_space(1))) void *gbl_vptr;
+
+// s_waitcnt immediate (gfx9 encoding) waiting until at most n vector memory
+// operations of this wave are outstanding; export and LDS counters untouched.
+__host__ __device__ constexpr int waitcnt_vm(int n) {
+  return (n & 15) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8);
+}
+
 template <typename R, typename V, int ORDER, int YC>
-__global__ void __launch_bounds__(256)
-    k_rotate_acc(const V *__restrict__ coef, int nb,
-                 const double *__restrict__ params, V *__restrict__ out,
-                 V *__restrict__ comp, int ny, int nz, int nx, int first,
-                 int last) {
-  const int x = blockIdx.x * 16 + threadIdx.x;
-  const int z = blockIdx.y * 16 + threadIdx.y;
-  const int y0 = blockIdx.z * YC;
-  if (x >= nx || z >= nz) return;
-  V sum[YC], cmp[YC];
+struct RotTile {
+  static constexpr int EC = sizeof(V) / 4;            // dwords per element
+  static constexpr int ROWD = kRotFoot * EC;          // dwords per tile row
+  static constexpr int PLD = kRotFoot * ROWD;         // dwords per tile plane
+  static constexpr int NW = 4;                        // waves per block
+  static constexpr int K = YC * PLD / 64 / NW;        // loads per wave/angle
+  static_assert(YC * PLD % (64 * NW) == 0, "tile must split evenly");
+  static_assert(K <= 63, "vmcnt range");
+
+  struct Box {
+    int bz0, bx0, H, W;
+    bool inner;
+  };
+
+  // footprint of the tile under angle p: bounding box of the mapped
+  // corners widened by the spline support and one sample against rounding
+  __device__ static __forceinline__ Box box(const double *p, double tz0,
+                                            double tz1, double tx0,
+                                            double tx1, int nz, int nx) {
+    const double m00 = p[0], m01 = p[1], m10 = p[2], m11 = p[3];
+    const double czlo = fmin(m00 * tz0, m00 * tz1) + fmin(m01 * tx0, m01 * tx1) + p[4];
+    const double czhi = fmax(m00 * tz0, m00 * tz1) + fmax(m01 * tx0, m01 * tx1) + p[4];
+    const double cxlo = fmin(m10 * tz0, m10 * tz1) + fmin(m11 * tx0, m11 * tx1) + p[5];
+    const double cxhi = fmax(m10 * tz0, m10 * tz1) + fmax(m11 * tx0, m11 * tx1) + p[5];
+    Box b;
+    b.bz0 = spline_start<ORDER>(fmax(czlo, 0.0)) - 1;
+    b.bx0 = spline_start<ORDER>(fmax(cxlo, 0.0)) - 1;
+    int H = spline_start<ORDER>(fmin(czhi, (double)(nz - 1))) + ORDER + 2 - b.bz0;
+    int W = spline_start<ORDER>(fmin(cxhi, (double)(nx - 1))) + ORDER + 2 - b.bx0;
+    b.H = H < 1 ? 1 : (H > kRotFoot ? kRotFoot : H);
+    b.W = W < 1 ? 1 : (W > kRotFoot ? kRotFoot : W);
+    b.inner = b.bz0 >= 0 && b.bz0 + b.H <= nz && b.bx0 >= 0 && b.bx0 + b.W <= nx;
+    return b;
+  }
+
+  // Direct global->LDS load of the footprint for YC planes: exactly K wave
+  // instructions per wave (a fixed, fully unrolled count, so vmcnt -- and
+  // the compiler's own wait insertion -- can separate the batches of
+  // consecutive angles).  Instruction k of the block writes tile dwords
+  // [64k, 64k+64).  A lane's tile column is the same for every instruction
+  // (computed and mirrored once); the row offset is uniform per wave.  Tile
+  // slots outside the footprint receive a clamped in-bounds sample that no
+  // tap reads.
+  __device__ static __forceinline__ void issue(const V *coef, const Box &b,
+                                               unsigned *dst, size_t plane,
+                                               int ny, int y0, int nz, int nx,
+                                               int wave, int lane) {
+    static_assert(EC == 1 || EC == 2 || EC == 4, "element size");
+    const unsigned *src = reinterpret_cast<const unsigned *>(coef);
+    // rows advanced per instruction, lane's sub-row and column
+    constexpr int RPI2 = EC == 4 ? 1 : 2 * (2 / EC);  // 2 x rows per instr
+    const int dr = EC == 1 ? lane >> 5 : 0;
+    int cc = EC == 1 ? lane & 31 : (EC == 2 ? lane >> 1 : lane >> 2);
+    const int d = lane & (EC - 1);
+    auto colofs = [&](int c) {
+      c = c < b.W ? c : b.W - 1;
+      const int col = b.inner ? b.bx0 + c : mirror_idx(b.bx0 + c, nx);
+      return col * EC + d;
+    };
+    const int lo0 = colofs(cc);
+    const int lo1 = EC == 4 ? colofs(cc + 16) : lo0;  // odd instructions
 #pragma unroll
-  for (int q = 0; q < YC; ++q) {
-    const int y = y0 + q;
-    if (!first && y < ny) {
-      const size_t o = ((size_t)z * ny + y) * nx + x;
-      sum[q] = out[o];
-      if (comp)
-        cmp[q] = comp[o];
-      else
-        acc_zero(cmp[q]);
-    } else {
-      acc_zero(sum[q]);
-      acc_zero(cmp[q]);
+    for (int i = 0; i < K; ++i) {
+      const int k = wave * K + i;
+      const int rl = (k * RPI2) / 2 + dr;  // row over all planes: q*32 + r
+      const int q = rl / kRotFoot;
+      int r = rl - q * kRotFoot;
+      r = r < b.H ? r : b.H - 1;
+      const int row = b.inner ? b.bz0 + r : mirror_idx(b.bz0 + r, nz);
+      const int qq = y0 + q < ny ? q : 0;
+      const int lo = (EC == 4 && (k & 1)) ? lo1 : lo0;
+      const unsigned *g = src + ((size_t)qq * plane + (size_t)row * nx) * EC + lo;
+      __builtin_amdgcn_global_load_lds((gbl_vptr)g, (lds_vptr)(dst + k * 64), 4,
+                                       0, 0);
     }
   }
-  const size_t plane = (size_t)nz * nx;
-  const double oz = (double)z, ox = (double)x;
-  for (int a = 0; a < nb; ++a) {
-    const double *p = params + 6 * a;
+
+  // gather and accumulate one angle from the tile in LDS
+  __device__ static __forceinline__ void compute(const V *tile, const Box &b,
+                                                 const double *p, double oz,
+                                                 double ox, int nz, int nx,
+                                                 wide_t<V> *sum) {
     const double cz = affine2(oz, ox, p[0], p[1], p[4]);
     const double cx = affine2(oz, ox, p[2], p[3], p[5]);
     if (cz < 0.0 || cz > (double)(nz - 1) || cx < 0.0 || cx > (double)(nx - 1))
-      continue;
+      return;
     R wz[ORDER + 1], wx[ORDER + 1];
-    int iz[ORDER + 1], ix[ORDER + 1];
     spline_weights<ORDER, R>(cz, wz);
     spline_weights<ORDER, R>(cx, wx);
-    const int sz = spline_start<ORDER>(cz), sx = spline_start<ORDER>(cx);
-#pragma unroll
-    for (int j = 0; j <= ORDER; ++j) {
-      iz[j] = mirror_idx(sz + j, nz) * nx;
-      ix[j] = mirror_idx(sx + j, nx);
-    }
-    const V *base = coef + (size_t)a * ny * plane;
+    int rz = spline_start<ORDER>(cz) - b.bz0;
+    int rx = spline_start<ORDER>(cx) - b.bx0;
+    // taps stay inside the footprint for rotations; the clamp keeps LDS
+    // reads in bounds for any other matrix
+    rz = rz < 0 ? 0 : (rz > kRotFoot - 1 - ORDER ? kRotFoot - 1 - ORDER : rz);
+    rx = rx < 0 ? 0 : (rx > kRotFoot - 1 - ORDER ? kRotFoot - 1 - ORDER : rx);
+    const V *t0 = tile + rz * kRotFoot + rx;
 #pragma unroll
     for (int q = 0; q < YC; ++q) {
-      const int y = y0 + q;
-      if (y >= ny) break;
-      const V *pl = base + (size_t)y * plane;
       V v;
       acc_zero(v);
 #pragma unroll
       for (int i = 0; i <= ORDER; ++i) {
-        V t;
-        acc_zero(t);
+        V tr;
+        acc_zero(tr);
 #pragma unroll
-        for (int j = 0; j <= ORDER; ++j) t = fma_w(wx[j], pl[iz[i] + ix[j]], t);
-        v = fma_w(wz[i], t, v);
+        for (int j = 0; j <= ORDER; ++j)
+          tr = fma_w(wx[j], t0[(q * kRotFoot + i) * kRotFoot + j], tr);
+        v = fma_w(wz[i], tr, v);
       }
-      acc_add(sum[q], cmp[q], v);
+      sum[q] = sum[q] + widen(v);
     }
   }
+};
+
+// Workgroup barrier without the memory fence of __syncthreads(): that fence
+// would also wait for the prefetch of the next tile.  Ordering is explicit
+// instead: each wave waits (vmcnt) for its own loads of the tile it is about
+// to read, and consumes its LDS reads of a tile before the barrier that
+// precedes the next prefetch into it.  The signal fences only stop the
+// compiler from moving memory operations across the barrier.
+__device__ __forceinline__ void barrier_no_fence() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+template <typename R, typename V, int ORDER, int YC>
+__global__ void __launch_bounds__(256)
+    k_rotate_lds(const V *__restrict__ coef, int nb,
+                 const double *__restrict__ params, V *__restrict__ out,
+                 V *__restrict__ comp, int ny, int nz, int nx, int first,
+                 int last, int ntx, int ntz, int xcd_map) {
+  using RT = RotTile<R, V, ORDER, YC>;
+  // two tile buffers as separate LDS objects: reads of one are not ordered
+  // behind the in-flight direct loads into the other
+  __shared__ __align__(16) V bufA[YC * kRotFoot * kRotFoot];
+  __shared__ __align__(16) V bufB[YC * kRotFoot * kRotFoot];
+  const int nt = ntx * ntz;
+  const int nyc = (ny + YC - 1) / YC;
+  int yc, ti;
+  if (xcd_map) {
+    const int bid = blockIdx.x;
+    yc = (bid & 7) + 8 * (bid / (8 * nt));
+    ti = (bid >> 3) % nt;
+    if (yc >= nyc) return;  // uniform per block
+  } else {
+    yc = blockIdx.x / nt;
+    ti = blockIdx.x - yc * nt;
+  }
+  const int tz = ti / ntx, tx = ti - tz * ntx;
+  const int lx = threadIdx.x & (kRotTile - 1), lz = threadIdx.x >> 4;
+  const int x = tx * kRotTile + lx, z = tz * kRotTile + lz;
+  const int y0 = yc * YC;
+  const bool own = x < nx && z < nz;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const size_t plane = (size_t)nz * nx;
+  wide_t<V> sum[YC];
 #pragma unroll
   for (int q = 0; q < YC; ++q) {
-    const int y = y0 + q;
-    if (y >= ny) break;
-    const size_t o = ((size_t)z * ny + y) * nx + x;
-    out[o] = sum[q];
-    if (!last && comp) comp[o] = cmp[q];
+    acc_zero(sum[q]);
+    if (!first && own && y0 + q < ny)
+      sum[q] = load_sum(out, comp, ((size_t)z * ny + y0 + q) * nx + x);
+  }
+  const double oz = (double)z, ox = (double)x;
+  const double tz0 = (double)(tz * kRotTile), tz1 = tz0 + (kRotTile - 1);
+  const double tx0 = (double)(tx * kRotTile), tx1 = tx0 + (kRotTile - 1);
+  unsigned *dA = reinterpret_cast<unsigned *>(bufA);
+  unsigned *dB = reinterpret_cast<unsigned *>(bufB);
+  const V *base = coef + (size_t)y0 * plane;
+  const size_t aStride = (size_t)ny * plane;
+  typename RT::Box bA = RT::box(params, tz0, tz1, tx0, tx1, nz, nx), bB;
+  RT::issue(base, bA, dA, plane, ny, y0, nz, nx, wave, lane);
+  for (int a = 0; a < nb; a += 2) {
+    // angle a from bufA, prefetch a+1 into bufB
+    if (a + 1 < nb) {
+      bB = RT::box(params + 6 * (a + 1), tz0, tz1, tx0, tx1, nz, nx);
+      RT::issue(base + (a + 1) * aStride, bB, dB, plane, ny, y0, nz, nx, wave,
+                lane);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    barrier_no_fence();
+    if (own) RT::compute(bufA, bA, params + 6 * a, oz, ox, nz, nx, sum);
+    barrier_no_fence();
+    if (a + 1 >= nb) break;
+    // angle a+1 from bufB, prefetch a+2 into bufA
+    if (a + 2 < nb) {
+      bA = RT::box(params + 6 * (a + 2), tz0, tz1, tx0, tx1, nz, nx);
+      RT::issue(base + (a + 2) * aStride, bA, dA, plane, ny, y0, nz, nx, wave,
+                lane);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    barrier_no_fence();
+    if (own) RT::compute(bufB, bB, params + 6 * (a + 1), oz, ox, nz, nx, sum);
+    barrier_no_fence();
+  }
+  if (own) {
+#pragma unroll
+    for (int q = 0; q < YC; ++q) {
+      if (y0 + q >= ny) break;
+      store_sum(out, comp, ((size_t)z * ny + y0 + q) * nx + x, sum[q], !last);
+    }
   }
 }
 
@@ -568,17 +785,11 @@ __global__ void __launch_bounds__(256)
   const int z = blockIdx.z;
   if (x >= nx || y >= ny) return;
   const size_t o = ((size_t)z * ny + y) * nx + x;
-  V sum, cmp;
-  if (!first) {
-    sum = out[o];
-    if (comp)
-      cmp = comp[o];
-    else
-      acc_zero(cmp);
-  } else {
+  wide_t<V> sum;
+  if (!first)
+    sum = load_sum(out, comp, o);
+  else
     acc_zero(sum);
-    acc_zero(cmp);
-  }
   const double o0 = (double)x, o1 = (double)(ny - 1 - y), o2 = (double)z;
   const size_t vol = (size_t)ny * nz * nx;
   for (int a = 0; a < nb; ++a) {
@@ -621,10 +832,9 @@ __global__ void __launch_bounds__(256)
       }
       v = fma_w(w1[q], tq, v);
     }
-    acc_add(sum, cmp, v);
+    sum = sum + widen(v);
   }
-  out[o] = sum;
-  if (!last && comp) comp[o] = cmp;
+  store_sum(out, comp, o, sum, !last);
 }
 
 // ---------------------------------------------------------------------------
@@ -671,7 +881,7 @@ __global__ void k_seq_rotate(R *data, long long njobs, int ny, int nz, int nx,
   }
 }
 
-// out[z][y][x] (+)= sum over the batch of slab[a][y][z][x], compensated.
+// out[z][y][x] (+)= sum over the batch of slab[a][y][z][x] (float64 sum).
 template <typename R, typename V>
 __global__ void __launch_bounds__(256)
     k_sum_angles(const V *__restrict__ slab, int nb, V *__restrict__ out,
@@ -683,21 +893,14 @@ __global__ void __launch_bounds__(256)
   const int x = (int)(v % nx);
   const long long zy = v / nx;
   const int y = (int)(zy % ny), z = (int)(zy / ny);
-  V sum, cmp;
-  if (!first) {
-    sum = out[v];
-    if (comp)
-      cmp = comp[v];
-    else
-      acc_zero(cmp);
-  } else {
+  wide_t<V> sum;
+  if (!first)
+    sum = load_sum(out, comp, (size_t)v);
+  else
     acc_zero(sum);
-    acc_zero(cmp);
-  }
   const long long src = ((long long)y * nz + z) * nx + x;
-  for (int a = 0; a < nb; ++a) acc_add(sum, cmp, slab[a * vol + src]);
-  out[v] = sum;
-  if (!last && comp) comp[v] = cmp;
+  for (int a = 0; a < nb; ++a) sum = sum + widen(slab[a * vol + src]);
+  store_sum(out, comp, (size_t)v, sum, !last);
 }
 
 // ---------------------------------------------------------------------------
@@ -830,38 +1033,46 @@ template <typename R, int N>
 static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
                                   const cpx<R> *twy, const double *pa,
                                   const PropAux<R> *aux, const R *q0z,
-                                  const R *q0y, const Geom &g, int na,
+                                  const R *q0y, const Geom &g, int na, int ldt,
                                   hipStream_t st) {
   constexpr int TT = N / 8;
-  const int threads = TT < 256 ? 256 : TT;
-  const int tpb = threads / TT;
-  const size_t sh = (2 * (size_t)tpb * lds_padded(N) + 4 * tpb) * sizeof(cpx<R>);
+  int tpb = 512 / TT < kColCpb ? 512 / TT : kColCpb;
+  if (tpb < 1) tpb = 1;
+  const size_t fft = ((size_t)Fft8<R, N, 1>::NBUF * tpb * lds_padded(N) + 4 * tpb) *
+                     sizeof(cpx<R>);
+  const size_t tile = (size_t)g.Ny * (kColCpb + 1) * sizeof(cpx<R>);
+  const int staged = fft + tile <= 96 * 1024 ? 1 : 0;
+  const size_t sh = fft + (staged ? tile : 0);
   if (sh > kLdsMax) return hipErrorInvalidValue;
   hipError_t e = set_lds((const void *)k_col_fft8<R, N>, sh);
   if (e != hipSuccess) return e;
-  const int kb = 16;
-  dim3 grid(g.Lx, (g.ln + kb - 1) / kb, na);
-  k_col_fft8<R, N><<<grid, threads, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g, kb);
+  const int kb = 8;
+  dim3 grid((g.Lx + kColCpb - 1) / kColCpb, (g.ln + kb - 1) / kb, na);
+  k_col_fft8<R, N><<<grid, tpb * TT, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g,
+                                               kb, ldt, staged);
   return hipGetLastError();
+}
+
+template <typename R, int N>
+static size_t row_lds(int tpb) {
+  return ((size_t)Fft8<R, N, 1>::NBUF * tpb * lds_padded(N) + 4 * tpb) * sizeof(cpx<R>);
 }
 
 template <typename R, typename V, int N>
 static hipError_t row_fft8_launch(const cpx<R> *T, V *slab, const cpx<R> *twx,
                                   const R *q0x, const Geom &g, int na,
-                                  int slab0, hipStream_t st) {
+                                  int slab0, int ldt, hipStream_t st) {
   constexpr int TT = N / 8;
   const int threads = TT < 256 ? 256 : TT;
   const int tpb = threads / TT;
-  const size_t fft = (2 * (size_t)tpb * lds_padded(N) + 4 * tpb) * sizeof(cpx<R>);
-  if (fft >= kLdsMax) return hipErrorInvalidValue;
-  int rb = (int)std::min<size_t>(16, (kLdsMax - fft) / ((size_t)g.Lx * sizeof(cpx<R>)));
-  if (rb > g.Ny) rb = g.Ny;
-  if (rb < 1) return hipErrorInvalidValue;
-  const size_t sh = fft + (size_t)g.Lx * rb * sizeof(cpx<R>);
+  const size_t sh = row_lds<R, N>(tpb);
+  if (sh > kLdsMax) return hipErrorInvalidValue;
   hipError_t e = set_lds((const void *)k_row_fft8<R, V, N>, sh);
   if (e != hipSuccess) return e;
+  int rb = 16 < g.Ny ? 16 : g.Ny;
   dim3 grid((g.Ny + rb - 1) / rb, g.ln, na);
-  k_row_fft8<R, V, N><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g, rb, slab0);
+  k_row_fft8<R, V, N><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g, rb,
+                                                 slab0, ldt);
   return hipGetLastError();
 }
 
@@ -869,61 +1080,76 @@ static hipError_t row_fft8_launch(const cpx<R> *T, V *slab, const cpx<R> *twx,
 template <typename R>
 static bool fft8_col_ok(const Geom &g) {
   if (g.logNy < 6 || g.logNy > 12) return false;
-  const int TT = g.lNy / 8, tpb = TT < 256 ? 256 / TT : 1;
-  return (2 * (size_t)tpb * lds_padded(g.lNy) + 4 * tpb) * sizeof(cpx<R>) <= kLdsMax;
+  const int TT = g.lNy / 8;
+  return (size_t)(TT > 64 ? 2 : 1) * lds_padded(g.lNy) * sizeof(cpx<R>) + 64 <= kLdsMax;
 }
 template <typename R>
 static bool fft8_row_ok(const Geom &g) {
   if (g.logNx < 6 || g.logNx > 12) return false;
-  const int TT = g.lNx / 8, tpb = TT < 256 ? 256 / TT : 1;
-  const size_t fft = (2 * (size_t)tpb * lds_padded(g.lNx) + 4 * tpb) * sizeof(cpx<R>);
-  return fft + (size_t)g.Lx * sizeof(cpx<R>) <= kLdsMax;
+  const int TT = g.lNx / 8;
+  return (size_t)(TT > 64 ? 2 : 1) * lds_padded(g.lNx) * sizeof(cpx<R>) + 64 <= kLdsMax;
 }
 
 template <typename R>
 hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
                            const double *pa, const PropAux<R> *aux,
                            const R *q0z, const R *q0y, const Geom &g, int na,
-                           int nb, hipStream_t st) {
+                           int nb, int ldt, hipStream_t st) {
   if (fft8_col_ok<R>(g)) {
     switch (g.lNy) {
-      case 64: return col_fft8_launch<R, 64>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
-      case 128: return col_fft8_launch<R, 128>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
-      case 256: return col_fft8_launch<R, 256>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
-      case 512: return col_fft8_launch<R, 512>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
-      case 1024: return col_fft8_launch<R, 1024>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
-      case 2048: return col_fft8_launch<R, 2048>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
-      case 4096: return col_fft8_launch<R, 4096>(S, T, twy, pa, aux, q0z, q0y, g, na, st);
+      case 64: return col_fft8_launch<R, 64>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
+      case 128: return col_fft8_launch<R, 128>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
+      case 256: return col_fft8_launch<R, 256>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
+      case 512: return col_fft8_launch<R, 512>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
+      case 1024: return col_fft8_launch<R, 1024>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
+      case 2048: return col_fft8_launch<R, 2048>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
+      case 4096: return col_fft8_launch<R, 4096>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
     }
   }
   const size_t sh = lds_bytes(g.lNy, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_col_ifft<R>, sh);
   if (e != hipSuccess) return e;
   dim3 grid(g.Lx, (g.ln + nb - 1) / nb, na);
-  k_col_ifft<R><<<grid, 256, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g, nb);
+  k_col_ifft<R><<<grid, 256, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g, nb, ldt);
   return hipGetLastError();
 }
 
 template <typename R, typename V>
 hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
                            const R *q0x, const Geom &g, int na, int nb,
-                           int slab0, hipStream_t st) {
+                           int slab0, int ldt, hipStream_t st) {
   if (fft8_row_ok<R>(g)) {
     switch (g.lNx) {
-      case 64: return row_fft8_launch<R, V, 64>(T, slab, twx, q0x, g, na, slab0, st);
-      case 128: return row_fft8_launch<R, V, 128>(T, slab, twx, q0x, g, na, slab0, st);
-      case 256: return row_fft8_launch<R, V, 256>(T, slab, twx, q0x, g, na, slab0, st);
-      case 512: return row_fft8_launch<R, V, 512>(T, slab, twx, q0x, g, na, slab0, st);
-      case 1024: return row_fft8_launch<R, V, 1024>(T, slab, twx, q0x, g, na, slab0, st);
-      case 2048: return row_fft8_launch<R, V, 2048>(T, slab, twx, q0x, g, na, slab0, st);
-      case 4096: return row_fft8_launch<R, V, 4096>(T, slab, twx, q0x, g, na, slab0, st);
+      case 64: return row_fft8_launch<R, V, 64>(T, slab, twx, q0x, g, na, slab0, ldt, st);
+      case 128: return row_fft8_launch<R, V, 128>(T, slab, twx, q0x, g, na, slab0, ldt, st);
+      case 256: return row_fft8_launch<R, V, 256>(T, slab, twx, q0x, g, na, slab0, ldt, st);
+      case 512: return row_fft8_launch<R, V, 512>(T, slab, twx, q0x, g, na, slab0, ldt, st);
+      case 1024: return row_fft8_launch<R, V, 1024>(T, slab, twx, q0x, g, na, slab0, ldt, st);
+      case 2048: return row_fft8_launch<R, V, 2048>(T, slab, twx, q0x, g, na, slab0, ldt, st);
+      case 4096: return row_fft8_launch<R, V, 4096>(T, slab, twx, q0x, g, na, slab0, ldt, st);
     }
   }
   const size_t sh = lds_bytes(g.lNx, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_row_ifft<R, V>, sh);
   if (e != hipSuccess) return e;
   dim3 grid((g.Ny + nb - 1) / nb, g.ln, na);
-  k_row_ifft<R, V><<<grid, 256, sh, st>>>(T, slab, twx, q0x, g, nb, slab0);
+  k_row_ifft<R, V><<<grid, 256, sh, st>>>(T, slab, twx, q0x, g, nb, slab0, ldt);
+  return hipGetLastError();
+}
+
+template <typename R, typename V, int ORDER, int YC>
+static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
+                                    V *out, V *comp, int ny, int nz, int nx,
+                                    int first, int last, hipStream_t st) {
+  const int ntx = (nx + kRotTile - 1) / kRotTile;
+  const int ntz = (nz + kRotTile - 1) / kRotTile;
+  const int nt = ntx * ntz;
+  const int nyc = (ny + YC - 1) / YC;
+  const int xcd_map = nyc >= 8 ? 1 : 0;
+  const long long blocks =
+      xcd_map ? 8LL * nt * ((nyc + 7) / 8) : (long long)nt * nyc;
+  k_rotate_lds<R, V, ORDER, YC><<<(unsigned)blocks, 256, 0, st>>>(
+      coef, nb, params, out, comp, ny, nz, nx, first, last, ntx, ntz, xcd_map);
   return hipGetLastError();
 }
 
@@ -931,17 +1157,11 @@ template <typename R, typename V, int ORDER>
 static hipError_t rotate_dispatch_yc(const V *coef, int nb, const double *params,
                                      V *out, V *comp, int ny, int nz, int nx,
                                      int first, int last, hipStream_t st) {
-  dim3 block(16, 16);
-  if (ny >= 8) {
-    dim3 grid((nx + 15) / 16, (nz + 15) / 16, (ny + 7) / 8);
-    k_rotate_acc<R, V, ORDER, 8><<<grid, block, 0, st>>>(
-        coef, nb, params, out, comp, ny, nz, nx, first, last);
-  } else {
-    dim3 grid((nx + 15) / 16, (nz + 15) / 16, ny);
-    k_rotate_acc<R, V, ORDER, 1><<<grid, block, 0, st>>>(
-        coef, nb, params, out, comp, ny, nz, nx, first, last);
-  }
-  return hipGetLastError();
+  if (ny >= 2)
+    return rotate_lds_launch<R, V, ORDER, 2>(coef, nb, params, out, comp, ny, nz,
+                                             nx, first, last, st);
+  return rotate_lds_launch<R, V, ORDER, 1>(coef, nb, params, out, comp, ny, nz,
+                                           nx, first, last, st);
 }
 
 template <typename R, typename V>
@@ -1103,15 +1323,15 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
                                          const Geom &, int, int, hipStream_t); \
   template hipError_t launch_col_ifft<R>(                                      \
       const cpx<R> *, cpx<R> *, const cpx<R> *, const double *,                \
-      const PropAux<R> *, const R *, const R *, const Geom &, int, int,        \
+      const PropAux<R> *, const R *, const R *, const Geom &, int, int, int,   \
       hipStream_t);                                                            \
   template hipError_t launch_row_ifft<R, R>(const cpx<R> *, R *,               \
                                             const cpx<R> *, const R *,         \
-                                            const Geom &, int, int, int,       \
+                                            const Geom &, int, int, int, int,  \
                                             hipStream_t);                      \
   template hipError_t launch_row_ifft<R, cpx<R>>(                              \
       const cpx<R> *, cpx<R> *, const cpx<R> *, const R *, const Geom &, int,  \
-      int, int, hipStream_t);                                                  \
+      int, int, int, hipStream_t);                                             \
   template hipError_t launch_rotate_acc<R, R>(int, const R *, int,             \
                                               const double *, R *, R *, int,   \
                                               int, int, int, int,              \

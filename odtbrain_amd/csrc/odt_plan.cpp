@@ -119,6 +119,7 @@ struct odt_plan {
   Geom g;
   bool explicit_iir;
   int NB, NC;                 // slabs per accumulate batch, angles per chunk
+  int ldt;                    // row stride of T (support columns, padded)
   int nb_fr, nb_fc, nb_col, nb_row;  // transforms per block
   size_t esz_c, esz_v;        // bytes of complex element / slab element
   size_t off_F1, off_S, off_T, off_slab, off_comp, ws_bytes;
@@ -189,8 +190,10 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   auto q0vec = [&](int n, bool on) {
     std::vector<R> q(std::max(n, 1), (R)0);
     if (on) {
+      // entries below 1e-20 are far under either precision's rounding of
+      // the O(1) terms they multiply; zero lets kernels skip them
       auto qd = prefilter_delta0(p->order, n);
-      for (int i = 0; i < n; ++i) q[i] = (R)qd[i];
+      for (int i = 0; i < n; ++i) q[i] = std::fabs(qd[i]) < 1e-20 ? (R)0 : (R)qd[i];
     }
     return q;
   };
@@ -260,10 +263,11 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
              "k_fwd_cols");
       HIPCHK(odt::launch_col_ifft<R>(S, T, (const cpx<R> *)p->twy, p->prop_a,
                                      (const PropAux<R> *)p->aux, (const R *)p->q0z,
-                                     (const R *)p->q0y, g, n, p->nb_col, st),
+                                     (const R *)p->q0y, g, n, p->nb_col, p->ldt, st),
              "k_col_ifft");
       HIPCHK((odt::launch_row_ifft<R, V>(T, slab, (const cpx<R> *)p->twx,
-                                         (const R *)p->q0x, g, n, p->nb_row, c0 - b0, st)),
+                                         (const R *)p->q0x, g, n, p->nb_row, c0 - b0,
+                                         p->ldt, st)),
              "k_row_ifft");
     }
     if (p->explicit_iir) {
@@ -390,7 +394,8 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   size_t budget = d->slab_budget_bytes > 0 ? (size_t)d->slab_budget_bytes
                                            : ((size_t)8 << 30);
   p->NB = (int)std::max<size_t>(1, std::min<size_t>(budget / slab_angle, (size_t)d->A));
-  const size_t t_angle = (size_t)g.ln * g.Lx * g.Ny * esz;
+  p->ldt = (g.Lx + 15) & ~15;
+  const size_t t_angle = (size_t)g.ln * p->ldt * g.Ny * esz;
   const size_t t_budget = (size_t)1 << 30;
   p->NC = (int)std::max<size_t>(1, std::min<size_t>(t_budget / t_angle, (size_t)p->NB));
   p->NC = std::min(p->NC, 64);
@@ -405,7 +410,7 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   p->off_S = off;
   off = align256(off + (size_t)p->NC * g.Lx * g.Ly * esz);
   p->off_T = off;
-  off = align256(off + (size_t)p->NC * g.ln * g.Lx * g.Ny * esz);
+  off = align256(off + (size_t)p->NC * g.ln * p->ldt * g.Ny * esz);
   p->off_slab = off;
   off = align256(off + (size_t)p->NB * slab_angle);
   p->off_comp = 0;
