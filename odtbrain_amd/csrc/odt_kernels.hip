@@ -730,10 +730,12 @@ __global__ void __launch_bounds__(256)
   unsigned *dB = reinterpret_cast<unsigned *>(bufB);
   const V *base = coef + (size_t)y0 * plane;
   const size_t aStride = (size_t)ny * plane;
-  typename RT::Box bA = RT::box(params, tz0, tz1, tx0, tx1, nz, nx), bB;
-  RT::issue(base, bA, dA, plane, ny, y0, nz, nx, wave, lane);
-  for (int a = 0; a < nb; a += 2) {
-    // angle a from bufA, prefetch a+1 into bufB
+  typename RT::Box bA, bB;
+  // Software pipeline over angles, two per iteration, every prefetch issued
+  // from the loop body: angle a+1 goes to B while a (in A) is computed, then
+  // a+2 goes to A while a+1 (in B) is computed.  The first iteration starts
+  // at a = -1 with nothing to compute.
+  for (int a = -1; a < nb; a += 2) {
     if (a + 1 < nb) {
       bB = RT::box(params + 6 * (a + 1), tz0, tz1, tx0, tx1, nz, nx);
       RT::issue(base + (a + 1) * aStride, bB, dB, plane, ny, y0, nz, nx, wave,
@@ -743,10 +745,9 @@ __global__ void __launch_bounds__(256)
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     }
     barrier_no_fence();
-    if (own) RT::compute(bufA, bA, params + 6 * a, oz, ox, nz, nx, sum);
+    if (a >= 0 && own) RT::compute(bufA, bA, params + 6 * a, oz, ox, nz, nx, sum);
     barrier_no_fence();
     if (a + 1 >= nb) break;
-    // angle a+1 from bufB, prefetch a+2 into bufA
     if (a + 2 < nb) {
       bA = RT::box(params + 6 * (a + 2), tz0, tz1, tx0, tx1, nz, nx);
       RT::issue(base + (a + 2) * aStride, bA, dA, plane, ny, y0, nz, nx, wave,
