@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device: a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
@@ -85,6 +88,12 @@ def main():
     from odtbrain_amd import _core, phantoms
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(2 ** 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log("bench: generating 256^2 x 360 ellipsoid Rytov sinogram")
     uB, angles = phantoms.ellipsoid_sinogram_3d(N=N_DET, A=N_ANG)
     d_in = torch.from_numpy(uB).to(dev)
@@ -173,7 +182,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(),
-                     "kernel": "k_rotate_acc", "ms_per_launch": rot_ms,
+                     "kernel": "k_rotate_lds", "ms_per_launch": rot_ms,
                      "angles_per_launch": ang_per_launch},
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,

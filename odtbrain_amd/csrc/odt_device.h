@@ -201,6 +201,63 @@ __device__ __forceinline__ void spline_weights(double c, W *w) {
   }
 }
 
+// Same weights from the fractional offset t = c - floor(c) (odd orders) or
+// t = c - floor(c + 0.5) (even orders), evaluated in W: the float32 path
+// forms t exactly in float64 and only the polynomials run in float32.
+template <int ORDER, typename W>
+__device__ __forceinline__ void spline_weights_t(W t, W *w) {
+  if (ORDER == 0) {
+    w[0] = W(1);
+  } else if (ORDER == 1) {
+    w[0] = W(1) - t;
+    w[1] = t;
+  } else if (ORDER == 2) {
+    const W a = W(0.5) - t;
+    w[0] = W(0.5) * a * a;
+    w[1] = W(0.75) - t * t;
+    w[2] = W(1) - w[0] - w[1];
+  } else if (ORDER == 3) {
+    const W z = W(1) - t;
+    w[0] = z * z * z / W(6);
+    w[1] = (t * t * (t - W(2)) * W(3) + W(4)) / W(6);
+    w[2] = (z * z * (z - W(2)) * W(3) + W(4)) / W(6);
+    w[3] = W(1) - w[0] - w[1] - w[2];
+  } else if (ORDER == 4) {
+    // distances 2+t, 1+t, |t|, 1-t (t in [-0.5, 0.5)); the last weight
+    // closes the partition of unity
+    const W d0 = W(2) + t, d1 = W(1) + t, d3 = W(1) - t;
+    const W ta = t < W(0) ? -t : t;
+    const W e0 = W(2.5) - d0;
+    w[0] = e0 * e0 * e0 * e0 / W(24);
+    w[1] = (W(55) + d1 * (W(20) + d1 * (W(-120) + d1 * (W(80) - W(16) * d1)))) / W(96);
+    const W t2 = ta * ta;
+    w[2] = t2 * (t2 * W(0.25) - W(0.625)) + W(115) / W(192);
+    w[3] = (W(55) + d3 * (W(20) + d3 * (W(-120) + d3 * (W(80) - W(16) * d3)))) / W(96);
+    w[4] = W(1) - w[0] - w[1] - w[2] - w[3];
+  } else {  // ORDER == 5, t in [0, 1): distances 2+t, 1+t, t, 1-t, 2-t
+    const W d[5] = {W(2) + t, W(1) + t, t, W(1) - t, W(2) - t};
+    W s = W(0);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const W x = d[j];
+      W v;
+      if (j == 0) {  // [2, 3)
+        const W e = W(3) - x;
+        v = e * e * e * e * e / W(120);
+      } else if (j == 2 || j == 3) {  // [0, 1]
+        const W x2 = x * x;
+        v = x2 * (x2 * (W(0.25) - x / W(12)) - W(0.5)) + W(0.55);
+      } else {  // [1, 2]
+        v = x * (x * (x * (x * (x / W(24) - W(0.375)) + W(1.25)) - W(1.75)) +
+                 W(0.625)) + W(0.425);
+      }
+      w[j] = v;
+      s += v;
+    }
+    w[5] = W(1) - s;
+  }
+}
+
 // Input coordinate of an affine map with scipy.ndimage's operation order:
 // ((0 + o0*m0) + o1*m1 [+ o2*m2]) + off, no fused multiply-add.
 __device__ __forceinline__ double affine2(double o0, double o1, double m0,

@@ -531,23 +531,33 @@ __device__ __forceinline__ cpx<R> fma_w(R w, cpx<R> v, cpx<R> acc) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged rotate-accumulate.  A block owns a 16x16 (z,x) output tile and
-// YC consecutive y planes.  Per angle it loads the input footprint of the
-// tile (bounding box of the rotated tile plus the spline support, mirror
-// extension applied while loading) for its YC planes into LDS with direct
-// global->LDS loads (global_load_lds: one wave instruction fills one 256-byte
-// tile row, no VGPR round trip), then every thread gathers its taps from
-// LDS.  The
-// input coordinates are computed exactly as scipy does (float64, no FMA), so
-// the out-of-bounds set is the reference's.  Sums over angles run in
-// float64 registers (float64 adds cost the same as float32 ones on this
-// GPU); between launches the float32 total carries a compensation term.
-// Blocks are numbered so that all tiles of one y-chunk land on the same XCD
-// (hardware dispatch is round-robin over the 8 XCDs): they share the
-// chunk's planes in that XCD's L2.
+// LDS-staged rotate-accumulate (_alg3d_bpp.py:73-81 rotates every y plane of
+// the filtered slab by -angle and sums over angles).
+//
+// A block (4 waves) owns a 16x16 (z,x) output tile and YC consecutive y
+// planes; wave w holds the 8x8 sub-tile (w>>1, w&1), its 16-lane group g the
+// 4x4 quad (g>>1, g&1) of that, lane l of the group the voxel (l>>2, l&3).  Per angle the block copies a 32x32 window of each of its
+// planes into LDS with direct global->LDS loads (16 bytes per lane when the
+// row pitch allows it; no VGPR round trip) and every thread gathers its
+// (ORDER+1)^2 taps from there.  k_rot_windows gives the window origin of
+// each (angle, tile); the window holds every tap, mirrored ones included, of
+// every in-bounds voxel of the tile.
+// LDS layout: window row r takes 32*sizeof(V) bytes; its 16-byte granules are
+// permuted g -> g ^ 2*(r mod 4).  The LDS serves 128 bytes per clock and
+// resolves bank conflicts within groups of 16 lanes; a group holds a 4x4
+// voxel quad, whose taps land on rotated 4x4 sample patterns, and the
+// permutation spreads those over the 32 banks for all rotation angles
+// (about 1.7 passes per 8-byte read on average instead of 4).
+// Input coordinates are computed exactly as scipy does (float64, no FMA), so
+// the out-of-bounds set and the tap positions are the reference's; weights,
+// tap sums and the sums over angles run in float64 (the float32 samples are
+// widened exactly); between launches the float32 total carries a
+// compensation term.  Blocks are numbered so that all tiles of one y-chunk
+// land on the same XCD (dispatch is round-robin over the 8 XCDs): they share
+// the chunk's planes in that XCD's L2.
 // ---------------------------------------------------------------------------
-constexpr int kRotTile = 16;   // output tile edge (z and x)
-constexpr int kRotFoot = 32;   // LDS footprint edge per plane (>= 15*sqrt2+8)
+constexpr int kRotTile = 16;  // output tile edge (z and x)
+constexpr int kRotWin = 32;   // LDS window edge per plane
 
 typedef __attribute__((address_space(3))) void *lds_vptr;
 typedef __attribute__((address_space(1))) void *gbl_vptr;
@@ -558,118 +568,224 @@ __host__ __device__ constexpr int waitcnt_vm(int n) {
   return (n & 15) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8);
 }
 
-template <typename R, typename V, int ORDER, int YC>
-struct RotTile {
-  static constexpr int EC = sizeof(V) / 4;            // dwords per element
-  static constexpr int ROWD = kRotFoot * EC;          // dwords per tile row
-  static constexpr int PLD = kRotFoot * ROWD;         // dwords per tile plane
-  static constexpr int NW = 4;                        // waves per block
-  static constexpr int K = YC * PLD / 64 / NW;        // loads per wave/angle
-  static_assert(YC * PLD % (64 * NW) == 0, "tile must split evenly");
-  static_assert(K <= 63, "vmcnt range");
+// First window row (or column) for the coordinate range [lo, hi] of a tile:
+// the spline start of the lowest in-range coordinate, one sample lower for
+// the zero-weight mirrored tap that odd orders touch at the upper edge,
+// aligned down to `al` samples and clamped into the plane.  The taps of the
+// tile then span at most 24 + ORDER samples (a 16-voxel tile edge covers at
+// most 15*sqrt(2) input samples), which leaves room for the alignment.
+template <int ORDER>
+__device__ __forceinline__ int rot_origin(double lo, double hi, int n, int al) {
+  lo = fmax(lo, 0.0);
+  hi = fmin(hi, (double)(n - 1));
+  int w = lo <= hi ? spline_start<ORDER>(lo) - 1 : 0;
+  w = w < 0 ? 0 : w - w % al;
+  const int wmax = n > kRotWin ? n - kRotWin : 0;
+  return w < wmax ? w : wmax;
+}
 
-  struct Box {
-    int bz0, bx0, H, W;
-    bool inner;
-  };
-
-  // footprint of the tile under angle p: bounding box of the mapped
-  // corners widened by the spline support and one sample against rounding
-  __device__ static __forceinline__ Box box(const double *p, double tz0,
-                                            double tz1, double tx0,
-                                            double tx1, int nz, int nx) {
-    const double m00 = p[0], m01 = p[1], m10 = p[2], m11 = p[3];
-    const double czlo = fmin(m00 * tz0, m00 * tz1) + fmin(m01 * tx0, m01 * tx1) + p[4];
-    const double czhi = fmax(m00 * tz0, m00 * tz1) + fmax(m01 * tx0, m01 * tx1) + p[4];
-    const double cxlo = fmin(m10 * tz0, m10 * tz1) + fmin(m11 * tx0, m11 * tx1) + p[5];
-    const double cxhi = fmax(m10 * tz0, m10 * tz1) + fmax(m11 * tx0, m11 * tx1) + p[5];
-    Box b;
-    b.bz0 = spline_start<ORDER>(fmax(czlo, 0.0)) - 1;
-    b.bx0 = spline_start<ORDER>(fmax(cxlo, 0.0)) - 1;
-    int H = spline_start<ORDER>(fmin(czhi, (double)(nz - 1))) + ORDER + 2 - b.bz0;
-    int W = spline_start<ORDER>(fmin(cxhi, (double)(nx - 1))) + ORDER + 2 - b.bx0;
-    b.H = H < 1 ? 1 : (H > kRotFoot ? kRotFoot : H);
-    b.W = W < 1 ? 1 : (W > kRotFoot ? kRotFoot : W);
-    b.inner = b.bz0 >= 0 && b.bz0 + b.H <= nz && b.bx0 >= 0 && b.bx0 + b.W <= nx;
-    return b;
-  }
-
-  // Direct global->LDS load of the footprint for YC planes: exactly K wave
-  // instructions per wave (a fixed, fully unrolled count, so vmcnt -- and
-  // the compiler's own wait insertion -- can separate the batches of
-  // consecutive angles).  Instruction k of the block writes tile dwords
-  // [64k, 64k+64).  A lane's tile column is the same for every instruction
-  // (computed and mirrored once); the row offset is uniform per wave.  Tile
-  // slots outside the footprint receive a clamped in-bounds sample that no
-  // tap reads.
-  __device__ static __forceinline__ void issue(const V *coef, const Box &b,
-                                               unsigned *dst, size_t plane,
-                                               int ny, int y0, int nz, int nx,
-                                               int wave, int lane) {
-    static_assert(EC == 1 || EC == 2 || EC == 4, "element size");
-    const unsigned *src = reinterpret_cast<const unsigned *>(coef);
-    // rows advanced per instruction, lane's sub-row and column
-    constexpr int RPI2 = EC == 4 ? 1 : 2 * (2 / EC);  // 2 x rows per instr
-    const int dr = EC == 1 ? lane >> 5 : 0;
-    int cc = EC == 1 ? lane & 31 : (EC == 2 ? lane >> 1 : lane >> 2);
-    const int d = lane & (EC - 1);
-    auto colofs = [&](int c) {
-      c = c < b.W ? c : b.W - 1;
-      const int col = b.inner ? b.bx0 + c : mirror_idx(b.bx0 + c, nx);
-      return col * EC + d;
-    };
-    const int lo0 = colofs(cc);
-    const int lo1 = EC == 4 ? colofs(cc + 16) : lo0;  // odd instructions
+// Window origins (z, x) of every (angle, tile).  A tile's computed input
+// coordinates are monotone in each output index (every rounding step is), so
+// the four corners, evaluated with the voxels' own float64 operations, span
+// them exactly.
+template <int ORDER>
+__global__ void k_rot_windows(const double *__restrict__ params, int nb, int ntz,
+                              int ntx, int nz, int nx, int al,
+                              int2 *__restrict__ win) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nt = ntz * ntx;
+  if (i >= (long long)nb * nt) return;
+  const int a = (int)(i / nt), ti = (int)(i - (long long)a * nt);
+  const int tz = ti / ntx, tx = ti - tz * ntx;
+  const double *p = params + 6 * (size_t)a;
+  const double z0 = tz * kRotTile, z1 = min(tz * kRotTile + kRotTile - 1, nz - 1);
+  const double x0 = tx * kRotTile, x1 = min(tx * kRotTile + kRotTile - 1, nx - 1);
+  double zlo = 0, zhi = 0, xlo = 0, xhi = 0;
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int k = wave * K + i;
-      const int rl = (k * RPI2) / 2 + dr;  // row over all planes: q*32 + r
-      const int q = rl / kRotFoot;
-      int r = rl - q * kRotFoot;
-      r = r < b.H ? r : b.H - 1;
-      const int row = b.inner ? b.bz0 + r : mirror_idx(b.bz0 + r, nz);
-      const int qq = y0 + q < ny ? q : 0;
-      const int lo = (EC == 4 && (k & 1)) ? lo1 : lo0;
-      const unsigned *g = src + ((size_t)qq * plane + (size_t)row * nx) * EC + lo;
-      __builtin_amdgcn_global_load_lds((gbl_vptr)g, (lds_vptr)(dst + k * 64), 4,
-                                       0, 0);
-    }
-  }
-
-  // gather and accumulate one angle from the tile in LDS
-  __device__ static __forceinline__ void compute(const V *tile, const Box &b,
-                                                 const double *p, double oz,
-                                                 double ox, int nz, int nx,
-                                                 wide_t<V> *sum) {
+  for (int c = 0; c < 4; ++c) {
+    const double oz = (c & 1) ? z1 : z0, ox = (c & 2) ? x1 : x0;
     const double cz = affine2(oz, ox, p[0], p[1], p[4]);
     const double cx = affine2(oz, ox, p[2], p[3], p[5]);
-    if (cz < 0.0 || cz > (double)(nz - 1) || cx < 0.0 || cx > (double)(nx - 1))
-      return;
-    R wz[ORDER + 1], wx[ORDER + 1];
-    spline_weights<ORDER, R>(cz, wz);
-    spline_weights<ORDER, R>(cx, wx);
-    int rz = spline_start<ORDER>(cz) - b.bz0;
-    int rx = spline_start<ORDER>(cx) - b.bx0;
-    // taps stay inside the footprint for rotations; the clamp keeps LDS
-    // reads in bounds for any other matrix
-    rz = rz < 0 ? 0 : (rz > kRotFoot - 1 - ORDER ? kRotFoot - 1 - ORDER : rz);
-    rx = rx < 0 ? 0 : (rx > kRotFoot - 1 - ORDER ? kRotFoot - 1 - ORDER : rx);
-    const V *t0 = tile + rz * kRotFoot + rx;
+    zlo = c ? fmin(zlo, cz) : cz;
+    zhi = c ? fmax(zhi, cz) : cz;
+    xlo = c ? fmin(xlo, cx) : cx;
+    xhi = c ? fmax(xhi, cx) : cx;
+  }
+  win[i] = make_int2(rot_origin<ORDER>(zlo, zhi, nz, 1),
+                     rot_origin<ORDER>(xlo, xhi, nx, al));
+}
+
+template <typename R>
+__device__ __forceinline__ R mul_w(R w, R v) { return w * v; }
+template <typename R>
+__device__ __forceinline__ cpx<R> mul_w(R w, cpx<R> v) {
+  return mk<R>(w * v.re, w * v.im);
+}
+
+template <typename R, typename V, int ORDER, int YC, int VEC>
+struct RotTile {
+  static constexpr int ELB = (int)sizeof(V);     // bytes per sample
+  static constexpr int ROWB = kRotWin * ELB;     // bytes per window row
+  static constexpr int NG = ROWB / 16;           // 16-byte granules per row
+  static constexpr int AL = 16 / ELB;            // samples per granule
+  static constexpr int PLB = kRotWin * ROWB;     // bytes per plane window
+  static constexpr int TB = YC * PLB;            // bytes per tile buffer
+  static constexpr int NW = 4;                   // waves per block
+  static constexpr int BPI = 256 * VEC;          // bytes per wave load
+  static constexpr int K = TB / BPI / NW;        // loads per wave per angle
+  static constexpr int RPI = BPI / ROWB;         // rows per wave load (VEC 4)
+  static constexpr int NLO = VEC == 4 ? (NG / RPI < K ? NG / RPI : K) : 1;
+  static_assert(ELB <= 16 && 16 % ELB == 0 && NG >= 8, "sample size");
+  static_assert(TB % (BPI * NW) == 0, "tile must split evenly");
+  static_assert(K <= 63, "vmcnt range");
+  static_assert(VEC == 1 || (RPI >= 1 && NG % RPI == 0), "row split");
+
+  // granule permutation of window row r
+  __device__ static __forceinline__ unsigned swz(int r) {
+    return ((unsigned)r & 3u) << 1;
+  }
+
+  // Per-lane byte offsets of the 16-byte loads (row within the load and
+  // permuted granule); they repeat every NLO loads.
+  __device__ static __forceinline__ void lane_offsets(unsigned *lo, int wave,
+                                                      int lane, int nx) {
+    if (VEC != 4) return;
+    const int rl = lane / NG, g = lane % NG;
+#pragma unroll
+    for (int i = 0; i < NLO; ++i) {
+      const int r = (wave * K + i) * RPI + rl;
+      lo[i] = (unsigned)(rl * nx + (int)(g ^ swz(r)) * AL) * ELB;
+    }
+  }
+
+  // Copy the windows of the block's planes for one angle into dst.  Load k
+  // (= wave*K + i) of the block fills tile bytes [k*BPI, (k+1)*BPI), lane l
+  // the 4*VEC bytes at k*BPI + 4*VEC*l.  16-byte loads are used only when
+  // both plane edges are >= 32, so windows lie inside the plane: the per-lane
+  // part of the address is fixed and the rest is uniform.  4-byte loads clamp
+  // rows and columns into the plane (slots outside it receive samples that no
+  // tap reads).  The two forms are separate instantiations: a branch between
+  // them lets the compiler merge their last loads, which loses the LDS
+  // object of the merged load and makes every later LDS read wait for all
+  // loads in flight.
+  __device__ static __forceinline__ void issue(const V *coefA, int wz0, int wx0,
+                                               char *dst, size_t plane, int nyv,
+                                               int nz, int nx, int wave,
+                                               int lane, const unsigned *lo) {
+    const char *src = reinterpret_cast<const char *>(coefA);
+    if constexpr (VEC == 4) {
+      const unsigned uo = (unsigned)(wz0 * nx + wx0) * ELB;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int k = wave * K + i;
+        const int rt = k * RPI;
+        const int q = rt / kRotWin, r = rt % kRotWin;
+        const int qq = q < nyv ? q : 0;
+        const char *g = src + ((size_t)qq * plane + (size_t)r * nx) * ELB;
+        __builtin_amdgcn_global_load_lds((gbl_vptr)(g + (lo[i % NLO] + uo)),
+                                         (lds_vptr)(dst + k * BPI), 16, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int k = wave * K + i;
+        const int b = k * BPI + lane * 4 * VEC;
+        const int rt = b / ROWB, wb = b % ROWB;
+        const int q = rt / kRotWin, r = rt % kRotWin;
+        const int sb = (int)(((unsigned)(wb >> 4) ^ swz(r)) << 4) + (wb & 15);
+        const int e = sb / ELB, eb = sb % ELB;
+        int row = wz0 + r;
+        row = row < nz ? row : nz - 1;
+        int col = wx0 + e;
+        col = col < nx - 1 ? col : nx - 1;
+        const int qq = q < nyv ? q : 0;
+        const char *g =
+            src + ((size_t)qq * plane + (size_t)row * nx + col) * ELB + eb;
+        __builtin_amdgcn_global_load_lds((gbl_vptr)g, (lds_vptr)(dst + k * BPI),
+                                         4, 0, 0);
+      }
+    }
+  }
+
+  __device__ static __forceinline__ V tap(const char *tile, unsigned off) {
+    return *reinterpret_cast<const V *>(tile + off);
+  }
+
+  // Window row of tap i for spline start s (plane edge n, window origin w0),
+  // mirrored as scipy's mirror mode does.  With 16-byte loads both plane
+  // edges are >= 32: taps of an in-range coordinate then leave the plane by
+  // less than n - 1 samples, where one reflection is the whole mirror rule,
+  // and below the centre tap only the lower edge can be crossed, above it
+  // only the upper one.  The 4-byte variant (any plane size) takes the
+  // general rule.
+  template <int I>
+  __device__ static __forceinline__ int tap_row(int s, int n, int w0) {
+    constexpr int C = ORDER / 2;  // centre tap of the spline support
+    const int t = s + I;
+    if constexpr (VEC == 1) return mirror_idx(t, n) - w0;
+    if (I < C) return (t < 0 ? -t : t) - w0;
+    if (I > C) {
+      const int d = n - 1 - t;
+      return (n - 1 - w0) - (d < 0 ? -d : d);
+    }
+    return t - w0;
+  }
+
+  // Gather one angle for the voxel (oz, ox) of every plane and add it to acc.
+  // Branch-free: a voxel outside the rotated plane (scipy's constant mode,
+  // value 0) or outside the volume gets zero weights and reads in-window
+  // samples.
+  __device__ static __forceinline__ void compute(const char *tile, int wz0,
+                                                 int wx0, const double *p,
+                                                 double oz, double ox, int nz,
+                                                 int nx, bool own,
+                                                 wide_t<V> *acc) {
+    const double cz = affine2(oz, ox, p[0], p[1], p[4]);
+    const double cx = affine2(oz, ox, p[2], p[3], p[5]);
+    const bool valid = own && cz >= 0.0 && cz <= (double)(nz - 1) && cx >= 0.0 &&
+                       cx <= (double)(nx - 1);
+    const double fz = (ORDER & 1) ? floor(cz) : floor(cz + 0.5);
+    const double fx = (ORDER & 1) ? floor(cx) : floor(cx + 0.5);
+    const int sz = valid ? (int)fz - ORDER / 2 : wz0 + 1;
+    const int sx = valid ? (int)fx - ORDER / 2 : wx0 + 1;
+    // weights and the tap sums in float64 (float64 FMAs issue at the float32
+    // rate on this GPU; the float32 samples are widened exactly)
+    double wz[ORDER + 1], wx[ORDER + 1], w[ORDER + 1][ORDER + 1];
+    spline_weights_t<ORDER, double>(cz - fz, wz);
+    spline_weights_t<ORDER, double>(cx - fx, wx);
+#pragma unroll
+    for (int i = 0; i <= ORDER; ++i)
+#pragma unroll
+      for (int j = 0; j <= ORDER; ++j) w[i][j] = valid ? wz[i] * wx[j] : 0.0;
+    int zr[ORDER + 1], xc[ORDER + 1];
+    rows_of<0>(sz, nz, wz0, zr);
+    rows_of<0>(sx, nx, wx0, xc);
+    unsigned rb[ORDER + 1], gs[ORDER + 1], cb[ORDER + 1];
+#pragma unroll
+    for (int i = 0; i <= ORDER; ++i) {
+      rb[i] = (unsigned)zr[i] * ROWB;
+      gs[i] = swz(zr[i]) << 4;
+      cb[i] = (unsigned)xc[i] * ELB;
+    }
 #pragma unroll
     for (int q = 0; q < YC; ++q) {
-      V v;
-      acc_zero(v);
+      wide_t<V> v =
+          mul_w(w[0][0], widen(tap(tile, rb[0] + (cb[0] ^ gs[0]) + q * PLB)));
 #pragma unroll
-      for (int i = 0; i <= ORDER; ++i) {
-        V tr;
-        acc_zero(tr);
+      for (int i = 0; i <= ORDER; ++i)
 #pragma unroll
         for (int j = 0; j <= ORDER; ++j)
-          tr = fma_w(wx[j], t0[(q * kRotFoot + i) * kRotFoot + j], tr);
-        v = fma_w(wz[i], tr, v);
-      }
-      sum[q] = sum[q] + widen(v);
+          if (i | j)
+            v = fma_w(w[i][j],
+                      widen(tap(tile, rb[i] + (cb[j] ^ gs[i]) + q * PLB)), v);
+      acc[q] = acc[q] + v;
     }
+  }
+
+  template <int I>
+  __device__ static __forceinline__ void rows_of(int s, int n, int w0, int *r) {
+    r[I] = tap_row<I>(s, n, w0);
+    if constexpr (I < ORDER) rows_of<I + 1>(s, n, w0, r);
   }
 };
 
@@ -685,17 +801,18 @@ __device__ __forceinline__ void barrier_no_fence() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-template <typename R, typename V, int ORDER, int YC>
+template <typename R, typename V, int ORDER, int YC, int VEC>
 __global__ void __launch_bounds__(256)
     k_rotate_lds(const V *__restrict__ coef, int nb,
-                 const double *__restrict__ params, V *__restrict__ out,
+                 const double *__restrict__ params,
+                 const int2 *__restrict__ win, V *__restrict__ out,
                  V *__restrict__ comp, int ny, int nz, int nx, int first,
                  int last, int ntx, int ntz, int xcd_map) {
-  using RT = RotTile<R, V, ORDER, YC>;
+  using RT = RotTile<R, V, ORDER, YC, VEC>;
   // two tile buffers as separate LDS objects: reads of one are not ordered
   // behind the in-flight direct loads into the other
-  __shared__ __align__(16) V bufA[YC * kRotFoot * kRotFoot];
-  __shared__ __align__(16) V bufB[YC * kRotFoot * kRotFoot];
+  __shared__ __align__(16) char bufA[RT::TB];
+  __shared__ __align__(16) char bufB[RT::TB];
   const int nt = ntx * ntz;
   const int nyc = (ny + YC - 1) / YC;
   int yc, ti;
@@ -709,61 +826,64 @@ __global__ void __launch_bounds__(256)
     ti = blockIdx.x - yc * nt;
   }
   const int tz = ti / ntx, tx = ti - tz * ntx;
-  const int lx = threadIdx.x & (kRotTile - 1), lz = threadIdx.x >> 4;
-  const int x = tx * kRotTile + lx, z = tz * kRotTile + lz;
-  const int y0 = yc * YC;
-  const bool own = x < nx && z < nz;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  const int z = tz * kRotTile + (wave >> 1) * 8 + ((lane >> 5) << 2) + ((lane >> 2) & 3);
+  const int x = tx * kRotTile + (wave & 1) * 8 + (((lane >> 4) & 1) << 2) + (lane & 3);
+  const int y0 = yc * YC;
+  const int nyv = ny - y0 < YC ? ny - y0 : YC;
+  const bool own = x < nx && z < nz;
   const size_t plane = (size_t)nz * nx;
+  unsigned lo[RT::NLO];
+  RT::lane_offsets(lo, wave, lane, nx);
   wide_t<V> sum[YC];
 #pragma unroll
   for (int q = 0; q < YC; ++q) {
     acc_zero(sum[q]);
-    if (!first && own && y0 + q < ny)
+    if (!first && own && q < nyv)
       sum[q] = load_sum(out, comp, ((size_t)z * ny + y0 + q) * nx + x);
   }
   const double oz = (double)z, ox = (double)x;
-  const double tz0 = (double)(tz * kRotTile), tz1 = tz0 + (kRotTile - 1);
-  const double tx0 = (double)(tx * kRotTile), tx1 = tx0 + (kRotTile - 1);
-  unsigned *dA = reinterpret_cast<unsigned *>(bufA);
-  unsigned *dB = reinterpret_cast<unsigned *>(bufB);
   const V *base = coef + (size_t)y0 * plane;
   const size_t aStride = (size_t)ny * plane;
-  typename RT::Box bA, bB;
+  const int2 *wt = win + ti;
+  int2 wA = make_int2(0, 0), wB = make_int2(0, 0);
   // Software pipeline over angles, two per iteration, every prefetch issued
   // from the loop body: angle a+1 goes to B while a (in A) is computed, then
   // a+2 goes to A while a+1 (in B) is computed.  The first iteration starts
   // at a = -1 with nothing to compute.
   for (int a = -1; a < nb; a += 2) {
     if (a + 1 < nb) {
-      bB = RT::box(params + 6 * (a + 1), tz0, tz1, tx0, tx1, nz, nx);
-      RT::issue(base + (a + 1) * aStride, bB, dB, plane, ny, y0, nz, nx, wave,
-                lane);
+      wB = wt[(size_t)(a + 1) * nt];
+      RT::issue(base + (a + 1) * aStride, wB.x, wB.y, bufB, plane, nyv, nz, nx,
+                wave, lane, lo);
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
     } else {
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     }
     barrier_no_fence();
-    if (a >= 0 && own) RT::compute(bufA, bA, params + 6 * a, oz, ox, nz, nx, sum);
+    if (a >= 0)
+      RT::compute(bufA, wA.x, wA.y, params + 6 * a, oz, ox, nz, nx, own, sum);
     barrier_no_fence();
-    if (a + 1 >= nb) break;
-    if (a + 2 < nb) {
-      bA = RT::box(params + 6 * (a + 2), tz0, tz1, tx0, tx1, nz, nx);
-      RT::issue(base + (a + 2) * aStride, bA, dA, plane, ny, y0, nz, nx, wave,
-                lane);
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
-    } else {
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    if (a + 1 < nb) {
+      if (a + 2 < nb) {
+        wA = wt[(size_t)(a + 2) * nt];
+        RT::issue(base + (a + 2) * aStride, wA.x, wA.y, bufA, plane, nyv, nz,
+                  nx, wave, lane, lo);
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+      } else {
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+      }
+      barrier_no_fence();
+      RT::compute(bufB, wB.x, wB.y, params + 6 * (a + 1), oz, ox, nz, nx, own,
+                  sum);
+      barrier_no_fence();
     }
-    barrier_no_fence();
-    if (own) RT::compute(bufB, bB, params + 6 * (a + 1), oz, ox, nz, nx, sum);
-    barrier_no_fence();
   }
   if (own) {
 #pragma unroll
     for (int q = 0; q < YC; ++q) {
-      if (y0 + q >= ny) break;
+      if (q >= nyv) break;
       store_sum(out, comp, ((size_t)z * ny + y0 + q) * nx + x, sum[q], !last);
     }
   }
@@ -1138,45 +1258,74 @@ hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
   return hipGetLastError();
 }
 
+size_t rotate_window_entries(int nz, int nx) {
+  return (size_t)((nz + kRotTile - 1) / kRotTile) * ((nx + kRotTile - 1) / kRotTile);
+}
+
+// Planes per block: 4 when both tile buffers then fit in 64 KiB, else 2.
+template <typename V>
+constexpr int rot_yc_max() {
+  return 2 * 4 * kRotWin * kRotWin * (int)sizeof(V) <= 65536 ? 4 : 2;
+}
+
 template <typename R, typename V, int ORDER, int YC>
 static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
-                                    V *out, V *comp, int ny, int nz, int nx,
-                                    int first, int last, hipStream_t st) {
+                                    int *win, V *out, V *comp, int ny, int nz,
+                                    int nx, int first, int last, hipStream_t st) {
   const int ntx = (nx + kRotTile - 1) / kRotTile;
   const int ntz = (nz + kRotTile - 1) / kRotTile;
   const int nt = ntx * ntz;
+  // 16-byte loads need windows inside the plane and 16-byte aligned rows
+  constexpr int AL = 16 / (int)sizeof(V);
+  const bool vec4 = nz >= kRotWin && nx >= kRotWin && nx % AL == 0 &&
+                    ((uintptr_t)coef & 15) == 0;
+  const long long nw = (long long)nb * nt;
+  k_rot_windows<ORDER><<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(
+      params, nb, ntz, ntx, nz, nx, vec4 ? AL : 1, (int2 *)win);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   const int nyc = (ny + YC - 1) / YC;
   const int xcd_map = nyc >= 8 ? 1 : 0;
   const long long blocks =
       xcd_map ? 8LL * nt * ((nyc + 7) / 8) : (long long)nt * nyc;
-  k_rotate_lds<R, V, ORDER, YC><<<(unsigned)blocks, 256, 0, st>>>(
-      coef, nb, params, out, comp, ny, nz, nx, first, last, ntx, ntz, xcd_map);
+  if (vec4)
+    k_rotate_lds<R, V, ORDER, YC, 4><<<(unsigned)blocks, 256, 0, st>>>(
+        coef, nb, params, (const int2 *)win, out, comp, ny, nz, nx, first, last,
+        ntx, ntz, xcd_map);
+  else
+    k_rotate_lds<R, V, ORDER, YC, 1><<<(unsigned)blocks, 256, 0, st>>>(
+        coef, nb, params, (const int2 *)win, out, comp, ny, nz, nx, first, last,
+        ntx, ntz, xcd_map);
   return hipGetLastError();
 }
 
 template <typename R, typename V, int ORDER>
 static hipError_t rotate_dispatch_yc(const V *coef, int nb, const double *params,
-                                     V *out, V *comp, int ny, int nz, int nx,
-                                     int first, int last, hipStream_t st) {
+                                     int *win, V *out, V *comp, int ny, int nz,
+                                     int nx, int first, int last, hipStream_t st) {
+  constexpr int YM = rot_yc_max<V>();
+  if (ny >= YM)
+    return rotate_lds_launch<R, V, ORDER, YM>(coef, nb, params, win, out, comp,
+                                              ny, nz, nx, first, last, st);
   if (ny >= 2)
-    return rotate_lds_launch<R, V, ORDER, 2>(coef, nb, params, out, comp, ny, nz,
-                                             nx, first, last, st);
-  return rotate_lds_launch<R, V, ORDER, 1>(coef, nb, params, out, comp, ny, nz,
-                                           nx, first, last, st);
+    return rotate_lds_launch<R, V, ORDER, 2>(coef, nb, params, win, out, comp,
+                                             ny, nz, nx, first, last, st);
+  return rotate_lds_launch<R, V, ORDER, 1>(coef, nb, params, win, out, comp, ny,
+                                           nz, nx, first, last, st);
 }
 
 template <typename R, typename V>
 hipError_t launch_rotate_acc(int order, const V *coef, int nb,
-                             const double *params, V *out, V *comp, int ny,
-                             int nz, int nx, int first, int last,
+                             const double *params, int *win, V *out, V *comp,
+                             int ny, int nz, int nx, int first, int last,
                              hipStream_t st) {
   switch (order) {
-    case 0: return rotate_dispatch_yc<R, V, 0>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 1: return rotate_dispatch_yc<R, V, 1>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 2: return rotate_dispatch_yc<R, V, 2>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 3: return rotate_dispatch_yc<R, V, 3>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 4: return rotate_dispatch_yc<R, V, 4>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 5: return rotate_dispatch_yc<R, V, 5>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 0: return rotate_dispatch_yc<R, V, 0>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 1: return rotate_dispatch_yc<R, V, 1>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 2: return rotate_dispatch_yc<R, V, 2>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 3: return rotate_dispatch_yc<R, V, 3>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 4: return rotate_dispatch_yc<R, V, 4>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 5: return rotate_dispatch_yc<R, V, 5>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
   }
   return hipErrorInvalidValue;
 }
@@ -1334,12 +1483,12 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
       const cpx<R> *, cpx<R> *, const cpx<R> *, const R *, const Geom &, int,  \
       int, int, int, hipStream_t);                                             \
   template hipError_t launch_rotate_acc<R, R>(int, const R *, int,             \
-                                              const double *, R *, R *, int,   \
-                                              int, int, int, int,              \
+                                              const double *, int *, R *, R *, \
+                                              int, int, int, int, int,         \
                                               hipStream_t);                    \
   template hipError_t launch_rotate_acc<R, cpx<R>>(                            \
-      int, const cpx<R> *, int, const double *, cpx<R> *, cpx<R> *, int, int,  \
-      int, int, int, hipStream_t);                                             \
+      int, const cpx<R> *, int, const double *, int *, cpx<R> *, cpx<R> *,     \
+      int, int, int, int, int, hipStream_t);                                   \
   template hipError_t launch_affine_acc<R, R>(int, const R *, int,             \
                                               const double *, R *, R *, int,   \
                                               int, int, int, int,              \

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -119,10 +120,11 @@ struct odt_plan {
   Geom g;
   bool explicit_iir;
   int NB, NC;                 // slabs per accumulate batch, angles per chunk
+  int RG;                     // angles per rotate-accumulate launch
   int ldt;                    // row stride of T (support columns, padded)
   int nb_fr, nb_fc, nb_col, nb_row;  // transforms per block
   size_t esz_c, esz_v;        // bytes of complex element / slab element
-  size_t off_F1, off_S, off_T, off_slab, off_comp, ws_bytes;
+  size_t off_F1, off_S, off_T, off_slab, off_comp, off_win, ws_bytes;
   // device tables
   void *twx = nullptr, *twy = nullptr, *P = nullptr, *aux = nullptr;
   void *q0x = nullptr, *q0y = nullptr, *q0z = nullptr;
@@ -305,9 +307,16 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
                                                g.Ny, g.ln, g.Nx, first, last, st)),
              "k_seq_rotate");
     else
-      HIPCHK((odt::launch_rotate_acc<R, V>(p->order, slab, nb, prm, out, comp, g.Ny,
-                                           g.ln, g.Nx, first, last, st)),
-             "k_rotate_acc");
+      for (int g0 = 0; g0 < nb; g0 += p->RG) {
+        const int gn = std::min(p->RG, nb - g0);
+        const size_t per_angle = (size_t)g.Ny * g.ln * g.Nx;
+        HIPCHK((odt::launch_rotate_acc<R, V>(
+                   p->order, slab + (size_t)g0 * per_angle, gn,
+                   prm + (size_t)g0 * p->pstride, (int *)(ws + p->off_win), out,
+                   comp, g.Ny, g.ln, g.Nx, first && g0 == 0,
+                   last && g0 + gn == nb, st)),
+               "k_rotate_acc");
+      }
     if (p->timing) HIPCHK(hipEventRecord(p->ev_rot[2 * p->n_rot + 1], st), "event");
     p->n_rot++;
   }
@@ -413,11 +422,26 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   off = align256(off + (size_t)p->NC * g.ln * p->ldt * g.Ny * esz);
   p->off_slab = off;
   off = align256(off + (size_t)p->NB * slab_angle);
+  // Angles per rotate-accumulate launch.  A block keeps its output tile in
+  // registers across the angles of one launch; blocks that share input
+  // planes drift apart over many angles and then no longer share them in
+  // L2, so long batches are split (the running sums go through the output
+  // and its compensation term between launches).
+  p->RG = p->NB;
+  if (d->kind == ODT_KIND_3D && d->intp_order >= 2) {
+    const char *env = std::getenv("ODTBRAIN_AMD_ROTATE_GROUP");
+    const int rg = env ? std::atoi(env) : 48;
+    if (rg > 0) p->RG = std::min(p->NB, rg);
+  }
   p->off_comp = 0;
-  if (p->NB < d->A) {
+  if (p->NB < d->A || p->RG < p->NB) {
     p->off_comp = off;
     off = align256(off + slab_angle);
   }
+  // window origins of the rotate-accumulate tiles (two ints per tile/angle)
+  p->off_win = off;
+  off = align256(off + 2 * sizeof(int) * (size_t)p->NB *
+                           odt::rotate_window_entries(p->g.ln, p->g.Nx));
   p->ws_bytes = off;
 
   int rc;
@@ -562,20 +586,35 @@ int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
   if (order < 0 || order > 5) return fail(ODT_ERR_INVALID, "order must be 0..5");
   if (nang < 1 || ny < 1 || nz < 1 || nx < 1 || !d_coef || !params || !d_out)
     return fail(ODT_ERR_INVALID, "bad arguments");
+  // the LDS windows of the tiles are sized for rotations: a row of the
+  // matrix may not stretch a tile edge beyond 15*sqrt(2) input samples
+  for (int a = 0; a < nang; ++a) {
+    const double *m = params + 6 * (size_t)a;
+    const double lim = std::sqrt(2.0) * (1 + 1e-12);
+    if (!(std::fabs(m[0]) + std::fabs(m[1]) <= lim) ||
+        !(std::fabs(m[2]) + std::fabs(m[3]) <= lim) || !std::isfinite(m[4]) ||
+        !std::isfinite(m[5]))
+      return fail(ODT_ERR_INVALID, "params must be rotation matrices");
+  }
   hipStream_t st = (hipStream_t)stream;
-  double *dp = nullptr;
-  HIPCHK(hipMalloc((void **)&dp, sizeof(double) * 6 * nang), "hipMalloc(params)");
+  // parameters followed by the window-origin scratch of the tiles
+  const size_t pbytes = align256(sizeof(double) * 6 * nang);
+  const size_t wbytes = 2 * sizeof(int) * (size_t)nang * odt::rotate_window_entries(nz, nx);
+  char *dp = nullptr;
+  HIPCHK(hipMalloc((void **)&dp, pbytes + wbytes), "hipMalloc(params)");
   hipError_t e = hipMemcpyAsync(dp, params, sizeof(double) * 6 * nang,
                                 hipMemcpyHostToDevice, st);
   if (e == hipSuccess) {
     const int first = accumulate ? 0 : 1;
+    const double *prm = (const double *)dp;
+    int *win = (int *)(dp + pbytes);
     e = precision == ODT_FP64
             ? odt::launch_rotate_acc<double, double>(order, (const double *)d_coef, nang,
-                                                     dp, (double *)d_out, nullptr, ny,
-                                                     nz, nx, first, 1, st)
+                                                     prm, win, (double *)d_out, nullptr,
+                                                     ny, nz, nx, first, 1, st)
             : odt::launch_rotate_acc<float, float>(order, (const float *)d_coef, nang,
-                                                   dp, (float *)d_out, nullptr, ny, nz,
-                                                   nx, first, 1, st);
+                                                   prm, win, (float *)d_out, nullptr, ny,
+                                                   nz, nx, first, 1, st);
   }
   hipError_t e2 = hipStreamSynchronize(st);
   (void)hipFree(dp);
