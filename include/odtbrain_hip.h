@@ -116,8 +116,9 @@ int odt_plan_execute(odt_plan *plan, const void *d_usin, void *d_out,
 int odt_plan_destroy(odt_plan *plan);
 
 /* Kernel timing of the last execute (milliseconds, HIP events on the plan's
- * stream), for the rotate-accumulate kernel and the whole call.  Timing is
- * recorded only when enabled. */
+ * stream), for the rotate-accumulate stage and the whole call, and the number
+ * of rotate-accumulate kernel launches the stage made.  Timing is recorded
+ * only when enabled. */
 int odt_plan_enable_timing(odt_plan *plan, int enable);
 int odt_plan_last_timing(const odt_plan *plan, double *rotate_ms,
                          double *filter_ms, double *total_ms,

@@ -133,7 +133,8 @@ struct odt_plan {
   bool timing = false;
   std::vector<hipEvent_t> ev_rot;  // start/stop pairs
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
-  int n_rot = 0;
+  int n_rot = 0;    // timed rotate stages (one per slab batch)
+  int n_rot_k = 0;  // rotate-accumulate kernel launches of the last execute
 };
 
 namespace {
@@ -253,6 +254,7 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
   const bool tilted = p->kind == ODT_KIND_3D_TILTED;
   if (p->timing) HIPCHK(hipEventRecord(p->ev_t0, st), "event");
   p->n_rot = 0;
+  p->n_rot_k = 0;
   for (int b0 = 0; b0 < p->A; b0 += p->NB) {
     const int nb = std::min(p->NB, p->A - b0);
     for (int c0 = b0; c0 < b0 + nb; c0 += p->NC) {
@@ -298,6 +300,7 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
       }
       HIPCHK(hipEventRecord(p->ev_rot[2 * p->n_rot], st), "event");
     }
+    p->n_rot_k += tilted || p->order < 2 ? 1 : (nb + p->RG - 1) / p->RG;
     if (tilted)
       HIPCHK((odt::launch_affine_acc<R, V>(p->order, slab, nb, prm, out, comp, g.Ny,
                                            g.ln, g.Nx, first, last, st)),
@@ -519,7 +522,7 @@ int odt_plan_last_timing(const odt_plan *p, double *rot_ms, double *filt_ms,
   if (rot_ms) *rot_ms = rot;
   if (total_ms) *total_ms = t;
   if (filt_ms) *filt_ms = t - rot;
-  if (launches) *launches = p->n_rot;
+  if (launches) *launches = p->n_rot_k;
   return ODT_OK;
 }
 
