@@ -142,7 +142,9 @@ int odt_spline_prefilter(int precision, int order, int64_t nlines,
  * d_coef: (nang, ny, nz, nx) coefficient planes (plane a,y has shape nz x nx),
  * d_out:  (nz, ny, nx), d_out[z,y,x] += sum_a interp(coef[a,y], M_a @ (z,x) + off_a)
  * with scipy.ndimage's constant-mode rule (outside [0,n-1] contributes 0).
- * params: host, nang*6 doubles (m00 m01 m10 m11 off0 off1).
+ * params: host, nang*6 doubles (m00 m01 m10 m11 off0 off1); each 2x2
+ * matrix must be a rotation (|m00|+|m01| and |m10|+|m11| at most sqrt(2),
+ * else ODT_ERR_INVALID): the kernel's LDS windows are sized for rotations.
  * accumulate: 0 overwrites d_out, 1 adds to it. */
 int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
                           int32_t nz, int32_t nx, const void *d_coef,

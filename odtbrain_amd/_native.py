@@ -9,6 +9,7 @@ import pathlib
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libodtbrain_hip.so"
 
 ODT_OK = 0
+ODT_ERR_INVALID, ODT_ERR_HIP, ODT_ERR_UNSUPPORTED, ODT_ERR_WORKSPACE = 1, 2, 3, 4
 ODT_KIND_3D, ODT_KIND_3D_TILTED, ODT_KIND_2D = 0, 1, 2
 ODT_FP32, ODT_FP64 = 0, 1
 ODT_PAD_NONE, ODT_PAD_EDGE, ODT_PAD_RAMP = 0, 1, 2

@@ -171,3 +171,14 @@ def test_plan_create_rejects_bad_descriptions():
         assert rc == 1, fields
         assert N.lib().odt_last_error()
         assert not h.value
+
+
+def test_rotate_accumulate_rejects_non_rotation():
+    """The standalone rotate entry point validates its matrices on the host
+    before any device work, so this runs without a GPU."""
+    params = np.array([[2.0, 0.0, 0.0, 1.0, 0.0, 0.0]])
+    dummy = ctypes.c_void_p(1)
+    rc = N.lib().odt_rotate_accumulate(N.ODT_FP32, 2, 1, 1, 8, 8, dummy,
+                                       N.dptr(params), dummy, 0, None)
+    assert rc == N.ODT_ERR_INVALID
+    assert b"rotation" in N.lib().odt_last_error()
