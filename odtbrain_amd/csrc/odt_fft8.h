@@ -76,6 +76,32 @@ __device__ __forceinline__ void dft8(cpx<F> *a) {
   a[7] = e3 - w3o3;
 }
 
+// dft8 with a[2..5] = 0 (first stage of a transform whose nonzero inputs lie
+// in the signed-frequency band (-N/4, N/4)); a[2..5] are not read.
+template <typename F, int DIR>
+__device__ __forceinline__ void dft8_band(cpx<F> *a) {
+  const cpx<F> m6 = mul_mi<F, DIR>(a[6]), m7 = mul_mi<F, DIR>(a[7]);
+  const cpx<F> e0 = a[0] + a[6], e2 = a[0] - a[6], e1 = a[0] - m6,
+               e3 = a[0] + m6;
+  const cpx<F> o0 = a[1] + a[7], o2 = a[1] - a[7], o1 = a[1] - m7,
+               o3 = a[1] + m7;
+  const F h = (F)0.70710678118654752440;
+  const cpx<F> w1o1 = DIR > 0 ? mk<F>(h * (o1.re - o1.im), h * (o1.re + o1.im))
+                              : mk<F>(h * (o1.re + o1.im), h * (o1.im - o1.re));
+  const cpx<F> w2o2 = mul_mi<F, DIR>(o2);
+  const cpx<F> w3o3 = DIR > 0
+                          ? mk<F>(-h * (o3.re + o3.im), h * (o3.re - o3.im))
+                          : mk<F>(h * (o3.im - o3.re), -h * (o3.re + o3.im));
+  a[0] = e0 + o0;
+  a[4] = e0 - o0;
+  a[1] = e1 + w1o1;
+  a[5] = e1 - w1o1;
+  a[2] = e2 + w2o2;
+  a[6] = e2 - w2o2;
+  a[3] = e3 + w3o3;
+  a[7] = e3 - w3o3;
+}
+
 template <typename F, int R, int DIR>
 __device__ __forceinline__ void dft_r(cpx<F> *a) {
   if (R == 8) {
@@ -99,7 +125,7 @@ struct Fft8 {
     return s < N8 ? 8 : (REM == 1 ? 2 : 4);
   }
 
-  template <int S, int L>
+  template <int S, int L, bool BAND>
   __device__ static __forceinline__ void stage(cpx<F> *v, cpx<F> *buf0,
                                                cpx<F> *buf1,
                                                const cpx<F> *__restrict__ tw,
@@ -112,11 +138,13 @@ struct Fft8 {
     cpx<F> *src = WAVE ? buf0 : ((S & 1) ? buf0 : buf1);  // from stage S-1
     cpx<F> *dst = WAVE ? buf0 : ((S & 1) ? buf1 : buf0);
     cpx<F> a[8];
+    constexpr bool PRUNED = BAND && FIRST && R == 8;
 #pragma unroll
     for (int i = 0; i < NB; ++i)
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        a[i * R + r] = FIRST ? v[i + r * NB] : src[lds_pad(j + i * T + r * (N / R))];
+        if (!PRUNED || r < 2 || r > 5)
+          a[i * R + r] = FIRST ? v[i + r * NB] : src[lds_pad(j + i * T + r * (N / R))];
     // wave mode: no compiler motion of this stage's LDS writes above its reads
     if (WAVE && !FIRST) __atomic_signal_fence(__ATOMIC_SEQ_CST);
 #pragma unroll
@@ -130,7 +158,10 @@ struct Fft8 {
           a[i * R + r] = a[i * R + r] * w;
         }
       }
-      dft_r<F, R, DIR>(a + i * R);
+      if constexpr (PRUNED)
+        dft8_band<F, DIR>(a + i * R);
+      else
+        dft_r<F, R, DIR>(a + i * R);
     }
     if (LAST) {
 #pragma unroll
@@ -153,12 +184,13 @@ struct Fft8 {
     }
   }
 
-  template <int S, int L>
+  template <int S, int L, bool BAND>
   __device__ static __forceinline__ void stages(cpx<F> *v, cpx<F> *b0,
                                                 cpx<F> *b1, const cpx<F> *tw,
                                                 int j) {
-    stage<S, L>(v, b0, b1, tw, j);
-    if constexpr (S + 1 < NS) stages<S + 1, L * radix(S)>(v, b0, b1, tw, j);
+    stage<S, L, BAND>(v, b0, b1, tw, j);
+    if constexpr (S + 1 < NS)
+      stages<S + 1, L * radix(S), BAND>(v, b0, b1, tw, j);
   }
 
   // Buffers one transform needs (PADN elements each).
@@ -167,12 +199,15 @@ struct Fft8 {
   // v[m] = x[j + m*T] on entry, X[j + m*T] on exit.  buf0 (and buf1 when
   // NBUF == 2): this transform's LDS scratch.  For NBUF == 2 all threads of
   // the block must call it (it contains barriers).
+  // BAND: the nonzero inputs lie in the signed-frequency band (-N/4, N/4),
+  // i.e. v[2..5] are zero (and are not read).
+  template <bool BAND = false>
   __device__ static __forceinline__ void run(cpx<F> *v, cpx<F> *buf0,
                                              cpx<F> *buf1, const cpx<F> *tw,
                                              int j) {
     // the previous call's last stage may still be reading the buffers
     if (NBUF == 2 && NS > 1) __syncthreads();
-    stages<0, 1>(v, buf0, buf1, tw, j);
+    stages<0, 1, BAND>(v, buf0, buf1, tw, j);
   }
 };
 

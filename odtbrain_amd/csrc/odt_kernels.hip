@@ -282,16 +282,26 @@ __device__ __forceinline__ void boundary_terms(const cpx<R> *v, int j, int p,
 }
 
 // Column stage, register radix-8 engine (power-of-two lNy = N).
-// A block owns CPB = 8 adjacent support columns c0..c0+7 and the slices
+// A block owns CPB = 16 adjacent support columns c0..c0+15 and the slices
 // k0..k0+kb-1; its tpb transform slots take the columns tpb at a time.  The
 // S column and the tables are read from cache per use (keeping them out of
-// the transform's registers).  Outputs are transposed through an LDS tile
-// so that each row of T[a][k][y][c] (stride ldt) is written in contiguous
-// 8-column segments; without room for the tile (very long columns) the
-// transform writes T directly.
-constexpr int kColCpb = 8;
+// the transform's registers).  Outputs are transposed through an LDS tile so
+// that each row of T[a][k][y][c] (stride ldt) is written in contiguous
+// 16-column (128-byte) segments; without room for the tile (very long
+// columns) the transform writes T directly.
+// BAND: the y support lies inside (-N/4, N/4), so of the eight register
+// slots (bins j + m*N/8) only m = 0, 1, 6, 7 hold inputs; the first radix-8
+// stage skips the others (half the loads and propagators).
+// Loads are predicated with selects on clamped indices, and when the crop
+// window is aligned to N/8 the crop test is uniform across the wavefront.
+constexpr int kColCpb = 16;
 
-template <typename R, int N>
+template <int N, bool BAND>
+__device__ __forceinline__ constexpr bool slot_used(int m) {
+  return !BAND || m < 2 || m > 5;
+}
+
+template <typename R, int N, bool BAND>
 __global__ void __launch_bounds__(512)
     k_col_fft8(const cpx<R> *__restrict__ S, cpx<R> *__restrict__ T,
                const cpx<R> *__restrict__ twy,
@@ -312,6 +322,17 @@ __global__ void __launch_bounds__(512)
   const int k0 = blockIdx.y * kb, ai = blockIdx.z;
   const int ln = g.ln, ny = g.Ny;
   const int kend = min(k0 + kb, ln);
+  // per-slot support index (clamped) and its validity: loop invariant
+  int cyc[8];
+  bool cyv[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int cy = freq_of_bin<N>(j + m * TT) - g.jy0;
+    cyv[m] = cy >= 0 && cy < g.Ly;
+    cyc[m] = cyv[m] ? cy : 0;
+  }
+  const bool uni = g.padyl % TT == 0 && ny % TT == 0;
+  const int mlo = g.padyl / TT, mhi = (g.padyl + ny) / TT;
   for (int k = k0; k < kend; ++k) {
     const double z = g.z0 + (double)k;
     // z-prefilter boundary terms only where q0 is not negligible
@@ -323,24 +344,25 @@ __global__ void __launch_bounds__(512)
       const int cl = cb + t;  // local column of this slot
       const int c = c0 + cl;
       const bool live = cl < kColCpb && c < g.Lx;
-      const cpx<R> *Sc = S + ((size_t)ai * g.Lx + c) * g.Ly;
-      const size_t tr = (size_t)c * g.Ly;
+      const int cs = live ? c : 0;
+      const cpx<R> *Sc = S + ((size_t)ai * g.Lx + cs) * g.Ly;
+      const size_t tr = (size_t)cs * g.Ly;
       cpx<R> v[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        v[m] = mk<R>(0, 0);
-        const int cy = freq_of_bin<N>(j + m * TT) - g.jy0;
-        if (live && cy >= 0 && cy < g.Ly) {
-          cpx<R> gk = phasor(prop_a[tr + cy] * z, R(0));
-          if (g.fuse_z) {
-            const PropAux<R> *x = aux + tr + cy;
-            gk = x->ib * gk;
-            if (edge) gk = gk + qa * x->r0 + qb * x->r1;
-          }
-          v[m] = Sc[cy] * gk;
+        if (!slot_used<N, BAND>(m)) continue;
+        const int cy = cyc[m];
+        cpx<R> gk = phasor(prop_a[tr + cy] * z, R(0));
+        if (g.fuse_z) {
+          const PropAux<R> *x = aux + tr + cy;
+          gk = x->ib * gk;
+          if (edge) gk = gk + qa * x->r0 + qb * x->r1;
         }
+        const cpx<R> val = Sc[cy] * gk;
+        const bool ok = live && cyv[m];
+        v[m] = mk<R>(ok ? val.re : R(0), ok ? val.im : R(0));
       }
-      FFT::run(v, b0, b1, twy, j);
+      FFT::template run<BAND>(v, b0, b1, twy, j);
       cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
       if (g.fuse_y)
         boundary_terms<R, N, TT, WAVE>(v, j, g.padyl, ny, (R)g.b1, corr, R0,
@@ -348,7 +370,8 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int y = j + m * TT - g.padyl;
-        if (live && y >= 0 && y < ny) {
+        const bool in = uni ? (m >= mlo && m < mhi) : (y >= 0 && y < ny);
+        if (live && in) {
           cpx<R> val = v[m];
           if (g.fuse_y) val = val + q0y[y] * R0 + q0y[ny - 1 - y] * R1;
           if (staged)
@@ -373,8 +396,9 @@ __global__ void __launch_bounds__(512)
 // Row stage, register radix-8 engine (power-of-two lNx = N).  Each
 // transform reads one row T[a][k][y][0..Lx) (contiguous, coalesced),
 // transforms it, crops with the fused x-prefilter terms and writes one slab
-// row.  Block: rows y0..y0+rb-1 of slice k, tpb rows at a time.
-template <typename R, typename V, int N>
+// row.  Block: rows y0..y0+rb-1 of slice k, tpb rows at a time.  BAND and
+// the load/crop predication as in the column stage.
+template <typename R, typename V, int N, bool BAND>
 __global__ void __launch_bounds__(512)
     k_row_fft8(const cpx<R> *__restrict__ T, V *__restrict__ slab,
                const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
@@ -389,21 +413,32 @@ __global__ void __launch_bounds__(512)
   cpx<R> *corr = lds + FFT::NBUF * tpb * FFT::PADN + 4 * t;
   const int y0 = blockIdx.x * rb, k = blockIdx.y, ai = blockIdx.z;
   const int ln = g.ln, ny = g.Ny, nx = g.Nx;
-  int cxm[8];
+  int cxc[8];
+  bool cxv[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) cxm[m] = freq_of_bin<N>(j + m * TT) - g.jx0;
+  for (int m = 0; m < 8; ++m) {
+    const int cx = freq_of_bin<N>(j + m * TT) - g.jx0;
+    cxv[m] = cx >= 0 && cx < g.Lx;
+    cxc[m] = cxv[m] ? cx : 0;
+  }
+  const bool uni = g.padxl % TT == 0 && nx % TT == 0;
+  const int mlo = g.padxl / TT, mhi = (g.padxl + nx) / TT;
   const int iters = (rb + tpb - 1) / tpb;
   for (int it = 0; it < iters; ++it) {
     const int r = it * tpb + t;
     const int y = y0 + r;
     const bool valid = r < rb && y < ny;
     if (WAVE && !valid) break;  // rows are taken in order: nothing left
-    const cpx<R> *src = T + (((size_t)ai * ln + k) * ny + y) * ldt;
+    const cpx<R> *src = T + (((size_t)ai * ln + k) * ny + (valid ? y : 0)) * ldt;
     cpx<R> v[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m)
-      v[m] = (valid && cxm[m] >= 0 && cxm[m] < g.Lx) ? src[cxm[m]] : mk<R>(0, 0);
-    FFT::run(v, b0, b1, twx, j);
+    for (int m = 0; m < 8; ++m) {
+      if (!slot_used<N, BAND>(m)) continue;
+      const cpx<R> val = src[cxc[m]];
+      const bool ok = valid && cxv[m];
+      v[m] = mk<R>(ok ? val.re : R(0), ok ? val.im : R(0));
+    }
+    FFT::template run<BAND>(v, b0, b1, twx, j);
     cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
     if (g.fuse_x)
       boundary_terms<R, N, TT, WAVE>(v, j, g.padxl, nx, (R)g.b1, corr, R0, R1);
@@ -412,7 +447,8 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int x = j + m * TT - g.padxl;
-        if (x >= 0 && x < nx) {
+        const bool in = uni ? (m >= mlo && m < mhi) : (x >= 0 && x < nx);
+        if (in) {
           cpx<R> val = v[m];
           if (g.fuse_x) val = val + q0x[x] * R0 + q0x[nx - 1 - x] * R1;
           store_val(dst + x, val);
@@ -1150,6 +1186,11 @@ hipError_t launch_fwd_cols(const cpx<R> *F1, cpx<R> *S, const cpx<R> *twy,
 
 static constexpr size_t kLdsMax = 160 * 1024;
 
+// support [j0, j0+len) of signed frequencies inside (-N/4, N/4)
+static bool in_band(int j0, int len, int N) {
+  return -j0 < N / 4 && j0 + len - 1 < N / 4;
+}
+
 template <typename R, int N>
 static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
                                   const cpx<R> *twy, const double *pa,
@@ -1157,7 +1198,9 @@ static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
                                   const R *q0y, const Geom &g, int na, int ldt,
                                   hipStream_t st) {
   constexpr int TT = N / 8;
-  int tpb = 512 / TT < kColCpb ? 512 / TT : kColCpb;
+  // 256-thread blocks: at ~150 VGPRs a SIMD holds three waves, so three
+  // 4-wave blocks fit per CU where only one 8-wave block would
+  int tpb = 256 / TT < kColCpb ? 256 / TT : kColCpb;
   if (tpb < 1) tpb = 1;
   const size_t fft = ((size_t)Fft8<R, N, 1>::NBUF * tpb * lds_padded(N) + 4 * tpb) *
                      sizeof(cpx<R>);
@@ -1165,12 +1208,19 @@ static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
   const int staged = fft + tile <= 96 * 1024 ? 1 : 0;
   const size_t sh = fft + (staged ? tile : 0);
   if (sh > kLdsMax) return hipErrorInvalidValue;
-  hipError_t e = set_lds((const void *)k_col_fft8<R, N>, sh);
-  if (e != hipSuccess) return e;
   const int kb = 8;
   dim3 grid((g.Lx + kColCpb - 1) / kColCpb, (g.ln + kb - 1) / kb, na);
-  k_col_fft8<R, N><<<grid, tpb * TT, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g,
-                                               kb, ldt, staged);
+  const bool band = in_band(g.jy0, g.Ly, N);
+  const void *fn = band ? (const void *)k_col_fft8<R, N, true>
+                        : (const void *)k_col_fft8<R, N, false>;
+  hipError_t e = set_lds(fn, sh);
+  if (e != hipSuccess) return e;
+  if (band)
+    k_col_fft8<R, N, true><<<grid, tpb * TT, sh, st>>>(S, T, twy, pa, aux, q0z,
+                                                       q0y, g, kb, ldt, staged);
+  else
+    k_col_fft8<R, N, false><<<grid, tpb * TT, sh, st>>>(S, T, twy, pa, aux, q0z,
+                                                        q0y, g, kb, ldt, staged);
   return hipGetLastError();
 }
 
@@ -1188,12 +1238,19 @@ static hipError_t row_fft8_launch(const cpx<R> *T, V *slab, const cpx<R> *twx,
   const int tpb = threads / TT;
   const size_t sh = row_lds<R, N>(tpb);
   if (sh > kLdsMax) return hipErrorInvalidValue;
-  hipError_t e = set_lds((const void *)k_row_fft8<R, V, N>, sh);
-  if (e != hipSuccess) return e;
   int rb = 16 < g.Ny ? 16 : g.Ny;
   dim3 grid((g.Ny + rb - 1) / rb, g.ln, na);
-  k_row_fft8<R, V, N><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g, rb,
-                                                 slab0, ldt);
+  const bool band = in_band(g.jx0, g.Lx, N);
+  const void *fn = band ? (const void *)k_row_fft8<R, V, N, true>
+                        : (const void *)k_row_fft8<R, V, N, false>;
+  hipError_t e = set_lds(fn, sh);
+  if (e != hipSuccess) return e;
+  if (band)
+    k_row_fft8<R, V, N, true><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g,
+                                                         rb, slab0, ldt);
+  else
+    k_row_fft8<R, V, N, false><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g,
+                                                          rb, slab0, ldt);
   return hipGetLastError();
 }
 

@@ -408,7 +408,10 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   p->NB = (int)std::max<size_t>(1, std::min<size_t>(budget / slab_angle, (size_t)d->A));
   p->ldt = (g.Lx + 15) & ~15;
   const size_t t_angle = (size_t)g.ln * p->ldt * g.Ny * esz;
-  const size_t t_budget = (size_t)1 << 30;
+  // Angles per filter chunk: the intermediate T of a chunk is written by the
+  // column stage and read back by the row stage.  Tuning knob (bytes).
+  const char *tenv = std::getenv("ODTBRAIN_AMD_T_BUDGET");
+  const size_t t_budget = tenv ? (size_t)std::atoll(tenv) : ((size_t)1 << 30);
   p->NC = (int)std::max<size_t>(1, std::min<size_t>(t_budget / t_angle, (size_t)p->NB));
   p->NC = std::min(p->NC, 64);
   p->nb_fr = per_block(g.lNx, esz, g.lNy);
