@@ -125,11 +125,50 @@ struct Fft8 {
     return s < N8 ? 8 : (REM == 1 ? 2 : 4);
   }
 
-  template <int S, int L, bool BAND>
+  // Twiddles of one thread, all stages: stage s >= 1 (radix R, L earlier
+  // points) needs w_{LR}^{r k} for r = 1..R-1 and each of its 8/R
+  // butterflies (k = b mod L, b = j + i*T).  They depend on j only, so a
+  // thread that runs many transforms loads them once (Twid, load_twiddles).
+  static __host__ __device__ constexpr int tw_count(int s) {
+    return s == 0 ? 0 : (8 / radix(s)) * (radix(s) - 1);
+  }
+  static __host__ __device__ constexpr int tw_off(int s) {
+    return s <= 1 ? 0 : tw_off(s - 1) + tw_count(s - 1);
+  }
+  static constexpr int NTW = tw_off(NS);
+  struct Twid {
+    cpx<F> w[NTW > 0 ? NTW : 1];
+  };
+
+  template <int S, int L>
+  __device__ static __forceinline__ void load_tw(Twid &t,
+                                                 const cpx<F> *__restrict__ tw,
+                                                 int j) {
+    constexpr int R = radix(S);
+    constexpr int NB = 8 / R;
+    if constexpr (S > 0) {
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int k = (j + i * T) & (L - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+          t.w[tw_off(S) + i * (R - 1) + r - 1] =
+              conj_if(tw[r * k * (N / (L * R))], DIR < 0);
+      }
+    }
+    if constexpr (S + 1 < NS) load_tw<S + 1, L * R>(t, tw, j);
+  }
+  __device__ static __forceinline__ void load_twiddles(Twid &t,
+                                                       const cpx<F> *tw,
+                                                       int j) {
+    load_tw<0, 1>(t, tw, j);
+  }
+
+  template <int S, int L, bool BAND, bool REG>
   __device__ static __forceinline__ void stage(cpx<F> *v, cpx<F> *buf0,
                                                cpx<F> *buf1,
                                                const cpx<F> *__restrict__ tw,
-                                               int j) {
+                                               const Twid *rt, int j) {
     constexpr int R = radix(S);
     constexpr int NB = 8 / R;
     constexpr bool FIRST = S == 0;
@@ -154,7 +193,8 @@ struct Fft8 {
       if (L > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const cpx<F> w = conj_if(tw[r * k * (N / (L * R))], DIR < 0);
+          const cpx<F> w = REG ? rt->w[tw_off(S) + i * (R - 1) + r - 1]
+                               : conj_if(tw[r * k * (N / (L * R))], DIR < 0);
           a[i * R + r] = a[i * R + r] * w;
         }
       }
@@ -184,13 +224,13 @@ struct Fft8 {
     }
   }
 
-  template <int S, int L, bool BAND>
+  template <int S, int L, bool BAND, bool REG>
   __device__ static __forceinline__ void stages(cpx<F> *v, cpx<F> *b0,
                                                 cpx<F> *b1, const cpx<F> *tw,
-                                                int j) {
-    stage<S, L, BAND>(v, b0, b1, tw, j);
+                                                const Twid *rt, int j) {
+    stage<S, L, BAND, REG>(v, b0, b1, tw, rt, j);
     if constexpr (S + 1 < NS)
-      stages<S + 1, L * radix(S), BAND>(v, b0, b1, tw, j);
+      stages<S + 1, L * radix(S), BAND, REG>(v, b0, b1, tw, rt, j);
   }
 
   // Buffers one transform needs (PADN elements each).
@@ -207,7 +247,15 @@ struct Fft8 {
                                              int j) {
     // the previous call's last stage may still be reading the buffers
     if (NBUF == 2 && NS > 1) __syncthreads();
-    stages<0, 1, BAND>(v, buf0, buf1, tw, j);
+    stages<0, 1, BAND, false>(v, buf0, buf1, tw, nullptr, j);
+  }
+  // Same with this thread's twiddles already in registers.
+  template <bool BAND = false>
+  __device__ static __forceinline__ void run(cpx<F> *v, cpx<F> *buf0,
+                                             cpx<F> *buf1, const Twid &t,
+                                             int j) {
+    if (NBUF == 2 && NS > 1) __syncthreads();
+    stages<0, 1, BAND, true>(v, buf0, buf1, nullptr, &t, j);
   }
 };
 

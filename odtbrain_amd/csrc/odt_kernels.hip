@@ -164,16 +164,31 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// propagator exp(i*phi), phi = km*(M-1)*z computed in float64 and reduced
-// modulo 2*pi before the single-precision sincos
+// propagator exp(i*phi), phi = km*(M-1)*z computed in float64.
+// Float32: the phase is reduced in float64 to quarter turns q and a
+// remainder x in [-pi/4, pi/4] (exact enough: |phi| < 2^20 here), then sin x
+// and cos x are Taylor polynomials through x^9 and x^8 (truncation below
+// 2e-9 on that interval), evaluated in float32 with FMA, and rotated by q
+// quarter turns (exact).  Max abs error below 1e-7 (4e-8 for sin, 9e-8
+// for cos when evaluated without FMA), against 1.3e-7 for the hardware
+// sin/cos; the float32 path's accuracy depends on it.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ cpx<float> phasor(double phi, float) {
-  // phase in revolutions, reduced to [-0.5, 0.5] in float64; the hardware
-  // sin/cos take revolutions (max abs error 1.3e-7 on this range, measured
-  // against float64 on gfx950 -- no worse than sincosf's 1.5e-7)
-  const double rev = phi * 0.15915494309189535;
-  const float r = (float)(rev - rint(rev));
-  return mk<float>(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+  const double qt = phi * 0.63661977236758134;  // phi / (pi/2)
+  const double qn = rint(qt);
+  const float x = (float)((qt - qn) * 1.5707963267948966);
+  const float x2 = x * x;
+  const float sn =
+      x + x * x2 *
+              (-1.6666667e-01f +
+               x2 * (8.3333333e-03f + x2 * (-1.9841270e-04f + x2 * 2.7557319e-06f)));
+  const float cs =
+      1.0f + x2 * (-0.5f + x2 * (4.1666668e-02f +
+                                 x2 * (-1.3888889e-03f + x2 * 2.4801587e-05f)));
+  const int q = (int)((long long)qn & 3);
+  return q == 0 ? mk<float>(cs, sn)
+                : q == 1 ? mk<float>(-sn, cs)
+                         : q == 2 ? mk<float>(-cs, -sn) : mk<float>(sn, -cs);
 }
 __device__ __forceinline__ cpx<double> phasor(double phi, double) {
   double s, c;
@@ -281,6 +296,67 @@ __device__ __forceinline__ void boundary_terms(const cpx<R> *v, int j, int p,
   if (WAVE) __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+__device__ __forceinline__ float rdlane(float x, int l) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+__device__ __forceinline__ double rdlane(double x, int l) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+// Value of lane `src` (an absolute lane of the wavefront) for every lane.
+__device__ __forceinline__ float permlane(float x, int src) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double permlane(double x, int src) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b & 0xffffffff));
+  const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+// Lane l of this transform's lane group (TT lanes, TT divides 64).
+template <int TT, typename R>
+__device__ __forceinline__ R group_lane(R x, int l) {
+  if constexpr (TT == 64) return rdlane(x, l);
+  else return permlane(x, (int)(threadIdx.x & (64 - TT)) + l);
+}
+// Output bin `bin` (the same for every transform of the wavefront) of a
+// transform held in the strided layout v[m] = X[j + m*TT] by TT lanes:
+// slot bin/TT of lane bin%TT of the group.
+template <typename R, int TT>
+__device__ __forceinline__ cpx<R> wave_bin(const cpx<R> *v, int bin) {
+  const int m = __builtin_amdgcn_readfirstlane(bin / TT);
+  const int l = bin - m * TT;
+  // uniform branches rather than x = v[m], which the compiler would turn
+  // into a dynamically indexed (scratch) array
+  switch (m) {
+    case 0: return mk<R>(group_lane<TT>(v[0].re, l), group_lane<TT>(v[0].im, l));
+    case 1: return mk<R>(group_lane<TT>(v[1].re, l), group_lane<TT>(v[1].im, l));
+    case 2: return mk<R>(group_lane<TT>(v[2].re, l), group_lane<TT>(v[2].im, l));
+    case 3: return mk<R>(group_lane<TT>(v[3].re, l), group_lane<TT>(v[3].im, l));
+    case 4: return mk<R>(group_lane<TT>(v[4].re, l), group_lane<TT>(v[4].im, l));
+    case 5: return mk<R>(group_lane<TT>(v[5].re, l), group_lane<TT>(v[5].im, l));
+    case 6: return mk<R>(group_lane<TT>(v[6].re, l), group_lane<TT>(v[6].im, l));
+    default: return mk<R>(group_lane<TT>(v[7].re, l), group_lane<TT>(v[7].im, l));
+  }
+}
+// Fused-prefilter boundary terms of a transform held by one wavefront (see
+// boundary_terms), read across lanes without LDS.
+template <typename R, int N, int TT>
+__device__ __forceinline__ void wave_boundary_terms(const cpx<R> *v, int p,
+                                                    int n, R b1, cpx<R> &R0,
+                                                    cpx<R> &R1) {
+  const cpx<R> y0 = wave_bin<R, TT>(v, (p - 1 + N) & (N - 1));
+  const cpx<R> y1 = wave_bin<R, TT>(v, (p + 1) & (N - 1));
+  const cpx<R> y2 = wave_bin<R, TT>(v, (p + n - 2 + N) & (N - 1));
+  const cpx<R> y3 = wave_bin<R, TT>(v, (p + n) & (N - 1));
+  R0 = b1 * (y0 - y1);
+  R1 = b1 * (y3 - y2);
+}
+
 // Column stage, register radix-8 engine (power-of-two lNy = N).
 // A block owns CPB = 16 adjacent support columns c0..c0+15 and the slices
 // k0..k0+kb-1; its tpb transform slots take the columns tpb at a time.  The
@@ -333,6 +409,8 @@ __global__ void __launch_bounds__(512)
   }
   const bool uni = g.padyl % TT == 0 && ny % TT == 0;
   const int mlo = g.padyl / TT, mhi = (g.padyl + ny) / TT;
+  typename FFT::Twid tw;
+  FFT::load_twiddles(tw, twy, j);
   for (int k = k0; k < kend; ++k) {
     const double z = g.z0 + (double)k;
     // z-prefilter boundary terms only where q0 is not negligible
@@ -362,11 +440,15 @@ __global__ void __launch_bounds__(512)
         const bool ok = live && cyv[m];
         v[m] = mk<R>(ok ? val.re : R(0), ok ? val.im : R(0));
       }
-      FFT::template run<BAND>(v, b0, b1, twy, j);
+      FFT::template run<BAND>(v, b0, b1, tw, j);
       cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
-      if (g.fuse_y)
-        boundary_terms<R, N, TT, WAVE>(v, j, g.padyl, ny, (R)g.b1, corr, R0,
-                                       R1);
+      if (g.fuse_y) {
+        if constexpr (WAVE)
+          wave_boundary_terms<R, N, TT>(v, g.padyl, ny, (R)g.b1, R0, R1);
+        else
+          boundary_terms<R, N, TT, WAVE>(v, j, g.padyl, ny, (R)g.b1, corr, R0,
+                                         R1);
+      }
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int y = j + m * TT - g.padyl;
@@ -423,6 +505,8 @@ __global__ void __launch_bounds__(512)
   }
   const bool uni = g.padxl % TT == 0 && nx % TT == 0;
   const int mlo = g.padxl / TT, mhi = (g.padxl + nx) / TT;
+  typename FFT::Twid tw;
+  FFT::load_twiddles(tw, twx, j);
   const int iters = (rb + tpb - 1) / tpb;
   for (int it = 0; it < iters; ++it) {
     const int r = it * tpb + t;
@@ -438,10 +522,15 @@ __global__ void __launch_bounds__(512)
       const bool ok = valid && cxv[m];
       v[m] = mk<R>(ok ? val.re : R(0), ok ? val.im : R(0));
     }
-    FFT::template run<BAND>(v, b0, b1, twx, j);
+    FFT::template run<BAND>(v, b0, b1, tw, j);
     cpx<R> R0 = mk<R>(0, 0), R1 = mk<R>(0, 0);
-    if (g.fuse_x)
-      boundary_terms<R, N, TT, WAVE>(v, j, g.padxl, nx, (R)g.b1, corr, R0, R1);
+    if (g.fuse_x) {
+      if constexpr (WAVE)
+        wave_boundary_terms<R, N, TT>(v, g.padxl, nx, (R)g.b1, R0, R1);
+      else
+        boundary_terms<R, N, TT, WAVE>(v, j, g.padxl, nx, (R)g.b1, corr, R0,
+                                       R1);
+    }
     if (valid) {
       V *dst = slab + (((size_t)(slab0 + ai) * ny + y) * ln + k) * nx;
 #pragma unroll
