@@ -124,6 +124,22 @@ int odt_plan_last_timing(const odt_plan *plan, double *rotate_ms,
                          double *filter_ms, double *total_ms,
                          int *rotate_launches);
 
+/* Batches of reconstructions replayed as one HIP graph.  The reference
+ * reconstructs one sinogram per call (and per Python process pool); a
+ * batch runs many plans of uneven sizes back to back on one stream:
+ * odt_batch_create executes every plan once (validation, warm-up), then
+ * captures the same executions into a graph; odt_batch_launch replays it.
+ * The buffers are fixed at creation: d_usin[i] / d_out[i] as for
+ * odt_plan_execute, one workspace shared by all entries (at least the
+ * largest odt_plan_workspace_size).  stream must be a created stream (the
+ * null stream cannot be captured); plans must not have timing enabled. */
+typedef struct odt_batch odt_batch;
+int odt_batch_create(int n, odt_plan *const *plans, const void *const *d_usin,
+                     void *const *d_out, void *d_ws, size_t ws_bytes,
+                     void *stream, odt_batch **batch);
+int odt_batch_launch(odt_batch *batch, void *stream);
+int odt_batch_destroy(odt_batch *batch);
+
 /* Batched 1D complex transforms along the last axis, unnormalised.
  * direction -1: forward (exp(-2 pi i jk/n)), +1: backward.  n arbitrary
  * (power of two: Stockham; otherwise direct DFT).  In place allowed. */

@@ -14,8 +14,9 @@ from ._alg2d_bpp import backpropagate_2d
 from ._alg3d_bpp import backpropagate_3d
 from ._alg3d_bppt import backpropagate_3d_tilted
 from . import util
+from .batch import Batch
 
 __version__ = "0.1.0"
 
 __all__ = ["backpropagate_2d", "backpropagate_3d", "backpropagate_3d_tilted",
-           "util"]
+           "util", "Batch"]

@@ -156,6 +156,31 @@ def device_input(uSin, complex_dtype):
     return torch.from_numpy(arr).cuda(), False
 
 
+class Collector:
+    """While active (``collect()``), ``run`` records each reconstruction
+    instead of executing it; ``odtbrain_amd.batch.Batch`` replays them."""
+
+    def __init__(self):
+        self.entries = []
+
+
+_collector = None
+
+
+class collect:
+    def __enter__(self):
+        global _collector
+        if _collector is not None:
+            raise RuntimeError("nested batch collection")
+        _collector = Collector()
+        return _collector
+
+    def __exit__(self, *exc):
+        global _collector
+        _collector = None
+        return False
+
+
 def run(desc_fields, weights, params, d_usin, out_shape, out_real,
         precision, return_tensor, timing=False):
     """Plan (cached), allocate output and workspace, execute."""
@@ -165,6 +190,11 @@ def run(desc_fields, weights, params, d_usin, out_shape, out_real,
         odt = torch.float64 if out_real else torch.complex128
     else:
         odt = torch.float32 if out_real else torch.complex64
+    if _collector is not None:
+        plan = Plan(desc_fields, weights, params, dev)  # owned by the batch
+        out = torch.empty(out_shape, dtype=odt, device=dev)
+        _collector.entries.append((plan, d_usin, out))
+        return out  # filled by every replay of the batch
     global last_plan
     plan = get_plan(desc_fields, weights, params, dev)
     last_plan = plan

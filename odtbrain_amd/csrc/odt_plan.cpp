@@ -504,6 +504,70 @@ int odt_plan_destroy(odt_plan *p) {
   return ODT_OK;
 }
 
+struct odt_batch {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int n = 0;
+};
+
+int odt_batch_create(int n, odt_plan *const *plans, const void *const *d_usin,
+                     void *const *d_out, void *ws, size_t ws_bytes,
+                     void *stream, odt_batch **out) {
+  if (!out || n < 1 || !plans || !d_usin || !d_out || !stream)
+    return fail(ODT_ERR_INVALID, "bad arguments (a non-null stream is required)");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!plans[i] || !d_usin[i] || !d_out[i])
+      return fail(ODT_ERR_INVALID, "null plan or buffer in batch");
+    if (plans[i]->timing) return fail(ODT_ERR_INVALID, "timing enabled in a batch plan");
+    if (ws_bytes < plans[i]->ws_bytes)
+      return fail(ODT_ERR_WORKSPACE, "workspace too small for batch entry " +
+                                         std::to_string(i));
+  }
+  hipStream_t st = (hipStream_t)stream;
+  // one eager pass: validates every entry and sets kernel attributes
+  // outside the capture
+  for (int i = 0; i < n; ++i) {
+    const int rc = odt_plan_execute(plans[i], d_usin[i], d_out[i], ws, ws_bytes, stream);
+    if (rc != ODT_OK) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(st), "batch warm-up");
+  odt_batch *b = new odt_batch;
+  b->n = n;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+  int rc = ODT_OK;
+  if (e == hipSuccess) {
+    for (int i = 0; i < n && rc == ODT_OK; ++i)
+      rc = odt_plan_execute(plans[i], d_usin[i], d_out[i], ws, ws_bytes, stream);
+    const hipError_t e2 = hipStreamEndCapture(st, &b->graph);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e == hipSuccess && rc == ODT_OK)
+    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess || rc != ODT_OK) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    delete b;
+    return rc != ODT_OK ? rc : hip_fail(e, "batch capture");
+  }
+  *out = b;
+  return ODT_OK;
+}
+
+int odt_batch_launch(odt_batch *b, void *stream) {
+  if (!b) return fail(ODT_ERR_INVALID, "null batch");
+  HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream), "hipGraphLaunch");
+  return ODT_OK;
+}
+
+int odt_batch_destroy(odt_batch *b) {
+  if (!b) return ODT_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  delete b;
+  return ODT_OK;
+}
+
 int odt_plan_enable_timing(odt_plan *p, int enable) {
   if (!p) return fail(ODT_ERR_INVALID, "null plan");
   p->timing = enable != 0;

@@ -125,3 +125,50 @@ def tilted_sphere_sinogram(N=192, A=240, count=5, coverage=1.4 * np.pi,
             ph += dn * 2.0 * np.sqrt(np.maximum(r * r - rho2, 0.0))
         out[i] = 1j * k0 * ph
     return out, angles, axis
+
+
+def disk_sinogram_2d(N=256, A=180, radius=None, dn=0.01, nm=1.333, res=6.0,
+                     offset=(0.0, 0.0), seed=0):
+    """Rytov sinogram (complex64, shape (A, N)) of a disk, uB = i*phase.
+
+    The disk (index nm + dn, centre offset from the rotation centre) is
+    projected analytically at angles uniform on [0, 2 pi); seeded phase
+    noise of 0.01 rad."""
+    rng = np.random.default_rng(seed)
+    r = N / 6.0 if radius is None else radius
+    phi = np.linspace(0, 2 * np.pi, A, endpoint=False)
+    xd = np.arange(N) - (N - 1) / 2.0
+    k0 = 2 * np.pi / res
+    # detector coordinate of the disk centre at each angle
+    cx, cz = offset
+    xc = cx * np.cos(phi) + cz * np.sin(phi)
+    rho2 = (xd[None, :] - xc[:, None]) ** 2
+    chord = 2 * np.sqrt(np.maximum(r * r - rho2, 0.0))
+    phase = k0 * dn * chord + rng.normal(0, 0.01, chord.shape)
+    return (1j * phase).astype(np.complex64), phi
+
+
+def batch_jobs(scale=1.0, seed=0):
+    """The batched workload: 32 2D sinograms (N in {128, 256, 384, 512,
+    1024}, 90-720 angles) and 16 3D sinograms (N 64-160, 60-300 angles),
+    seeded disk/sphere phantoms with dn in [0.005, 0.02], complex64.
+    Returns a list of (kind, uSin, angles, res, nm).  ``scale`` < 1 shrinks
+    the angle counts (for tests)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    sizes2 = [128, 256, 384, 512, 1024]
+    for i in range(32):
+        N = sizes2[i % len(sizes2)]
+        A = max(8, int(rng.integers(90, 721) * scale))
+        dn = float(rng.uniform(0.005, 0.02))
+        s, ang = disk_sinogram_2d(N=N, A=A, dn=dn, seed=seed + i)
+        jobs.append(("2d", s, ang, 6.0, 1.333))
+    for i in range(16):
+        N = int(rng.integers(64, 161))
+        A = max(8, int(rng.integers(60, 301) * scale))
+        r = float(rng.uniform(0.15, 0.3) * N)
+        n_s = 1.333 + float(rng.uniform(0.005, 0.02))
+        s, ang = sphere_sinogram_3d(N=N, A=A, radius=r, n_sphere=n_s,
+                                    seed=seed + 100 + i)
+        jobs.append(("3d", s.astype(np.complex64), ang, 6.0, 1.333))
+    return jobs
