@@ -124,6 +124,16 @@ int odt_plan_last_timing(const odt_plan *plan, double *rotate_ms,
                          double *filter_ms, double *total_ms,
                          int *rotate_launches);
 
+/* Adjoint of odt_plan_execute: d_gusin (A, Ny, Nx) = M^H d_gout, where M is
+ * the (complex-linear) map from the sinogram to the volume that execute
+ * applies.  There is no counterpart in the reference, which has no
+ * gradient; it is the backward pass of a differentiable reconstruction.
+ * Supported: ODT_KIND_3D plans with complex output, ODT_FLAG_IIR_PREFILTER,
+ * spline order 2..5, edge or no padding, padded sizes 64..512.  Uses the
+ * plan's workspace (same size as execute). */
+int odt_plan_adjoint(odt_plan *plan, const void *d_gout, void *d_gusin,
+                     void *d_ws, size_t ws_bytes, void *stream);
+
 /* Batches of reconstructions replayed as one HIP graph.  The reference
  * reconstructs one sinogram per call (and per Python process pool); a
  * batch runs many plans of uneven sizes back to back on one stream:

@@ -15,8 +15,10 @@ from ._alg3d_bpp import backpropagate_3d
 from ._alg3d_bppt import backpropagate_3d_tilted
 from . import util
 from .batch import Batch
+from .autograd import backpropagate_3d_adjoint, backpropagate_3d_torch
 
 __version__ = "0.1.0"
 
 __all__ = ["backpropagate_2d", "backpropagate_3d", "backpropagate_3d_tilted",
-           "util", "Batch"]
+           "util", "Batch", "backpropagate_3d_adjoint",
+           "backpropagate_3d_torch"]

@@ -75,6 +75,30 @@ def _as_input(uSin, copy):
     return np.array(uSin, copy=copy)
 
 
+def _plan_fields(angles, res, nm, lD, lny, lnx, padyl, padxl, lNy, lNx,
+                 pad_mode, pv_re, pv_im, intp_order, onlyreal, prefilter,
+                 precision, device):
+    """Plan description and rotation parameters (host float64)."""
+    A = angles.size
+    ln = lnx
+    km = (2 * np.pi * nm) / res
+    # scipy.ndimage.rotate(arr, angle=-phi0, axes=(0, 2)) per y-plane
+    # (_alg3d_bpp.py:73-81, :534)
+    phi0 = np.rad2deg(np.asarray(angles, dtype=float).reshape(-1))
+    params = _core.planar_params([-p for p in phi0], ln, lnx)
+    fields = dict(
+        kind=N.ODT_KIND_3D, precision=precision, A=A, Ny=lny, Nx=lnx,
+        lNy=lNy, lNx=lNx, padyl=padyl, padxl=padxl, pad_mode=pad_mode,
+        intp_order=int(intp_order), onlyreal=int(bool(onlyreal)),
+        flags=N.ODT_FLAG_IIR_PREFILTER if prefilter == "iir" else 0,
+        angle_param_stride=6, padval_re=pv_re, padval_im=pv_im,
+        km=float(km), lD=float(lD), z0=-ln / 2.0, ramp_u=0.0, ramp_v=1.0)
+    esz = (4 if precision == N.ODT_FP32 else 8) * (1 if onlyreal else 2)
+    fields["slab_budget_bytes"] = _core.slab_budget(
+        ln * lny * lnx * esz, A, device)
+    return fields, params
+
+
 def backpropagate_3d(uSin, angles, res, nm, lD=0, coords=None,
                      weight_angles=True, onlyreal=False,
                      padding=(True, True), padfac=1.75, padval="edge",
@@ -124,22 +148,11 @@ def backpropagate_3d(uSin, angles, res, nm, lD=0, coords=None,
             weights = np.ones(A)
     else:
         weights = np.ones(A)
-    km = (2 * np.pi * nm) / res
-    # scipy.ndimage.rotate(arr, angle=-phi0, axes=(0, 2)) per y-plane
-    # (_alg3d_bpp.py:73-81, :534)
-    phi0 = np.rad2deg(np.asarray(angles, dtype=float).reshape(-1))
-    params = _core.planar_params([-p for p in phi0], ln, lnx)
     d_usin, was_tensor = _core.device_input(uSin, cdt)
-    fields = dict(
-        kind=N.ODT_KIND_3D, precision=precision, A=A, Ny=lny, Nx=lnx,
-        lNy=lNy, lNx=lNx, padyl=padyl, padxl=padxl, pad_mode=pad_mode,
-        intp_order=int(intp_order), onlyreal=int(bool(onlyreal)),
-        flags=N.ODT_FLAG_IIR_PREFILTER if prefilter == "iir" else 0,
-        angle_param_stride=6, padval_re=pv_re, padval_im=pv_im,
-        km=float(km), lD=float(lD), z0=-ln / 2.0, ramp_u=0.0, ramp_v=1.0)
-    esz = (4 if precision == N.ODT_FP32 else 8) * (1 if onlyreal else 2)
-    fields["slab_budget_bytes"] = _core.slab_budget(
-        ln * lny * lnx * esz, A, d_usin.device)
+    fields, params = _plan_fields(
+        angles, res, nm, lD, lny, lnx, padyl, padxl, lNy, lNx, pad_mode,
+        pv_re, pv_im, intp_order, onlyreal, prefilter, precision,
+        d_usin.device)
     if count is not None:
         count.value += 2
     out = _core.run(fields, weights, params, d_usin, (ln, lny, lnx),

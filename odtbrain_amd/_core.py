@@ -100,6 +100,22 @@ class Plan:
         return {"rotate_ms": r.value, "filter_ms": f.value,
                 "total_ms": t.value, "rotate_launches": n.value}
 
+    def adjoint(self, d_gout, d_gusin, workspace=None, stream=None):
+        """d_gusin = M^H d_gout for the sinogram-to-volume map M of this
+        plan (odt_plan_adjoint)."""
+        torch = _torch()
+        if workspace is None:
+            workspace = torch.empty(max(1, self.workspace_bytes),
+                                    dtype=torch.uint8, device=d_gout.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(d_gout.device).cuda_stream
+        N.check(N.lib().odt_plan_adjoint(
+            self.handle, ctypes.c_void_p(d_gout.data_ptr()),
+            ctypes.c_void_p(d_gusin.data_ptr()),
+            ctypes.c_void_p(workspace.data_ptr()), self.workspace_bytes,
+            ctypes.c_void_p(stream)), "odt_plan_adjoint")
+        return d_gusin
+
     def execute(self, d_usin, d_out, workspace=None, stream=None):
         torch = _torch()
         if workspace is None:

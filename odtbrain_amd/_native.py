@@ -19,6 +19,7 @@ EXPORTED = (
     "odt_plan_create", "odt_plan_workspace_size", "odt_plan_execute",
     "odt_plan_destroy", "odt_plan_enable_timing", "odt_plan_last_timing",
     "odt_fft_c2c", "odt_spline_prefilter", "odt_rotate_accumulate",
+    "odt_plan_adjoint",
     "odt_batch_create", "odt_batch_launch", "odt_batch_destroy",
     "odt_last_error", "odt_version",
 )
@@ -85,6 +86,7 @@ def lib():
     L.odt_rotate_accumulate.argtypes = [ctypes.c_int, ctypes.c_int, i32, i32,
                                         i32, i32, vp, dp, vp, ctypes.c_int,
                                         vp]
+    L.odt_plan_adjoint.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp]
     vpp = ctypes.POINTER(vp)
     L.odt_batch_create.argtypes = [ctypes.c_int, vpp, vpp, vpp, vp,
                                    ctypes.c_size_t, vp, vpp]

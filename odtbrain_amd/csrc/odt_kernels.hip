@@ -163,39 +163,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// ---------------------------------------------------------------------------
-// propagator exp(i*phi), phi = km*(M-1)*z computed in float64.
-// Float32: the phase is reduced in float64 to quarter turns q and a
-// remainder x in [-pi/4, pi/4] (exact enough: |phi| < 2^20 here), then sin x
-// and cos x are Taylor polynomials through x^9 and x^8 (truncation below
-// 2e-9 on that interval), evaluated in float32 with FMA, and rotated by q
-// quarter turns (exact).  Max abs error below 1e-7 (4e-8 for sin, 9e-8
-// for cos when evaluated without FMA), against 1.3e-7 for the hardware
-// sin/cos; the float32 path's accuracy depends on it.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ cpx<float> phasor(double phi, float) {
-  const double qt = phi * 0.63661977236758134;  // phi / (pi/2)
-  const double qn = rint(qt);
-  const float x = (float)((qt - qn) * 1.5707963267948966);
-  const float x2 = x * x;
-  const float sn =
-      x + x * x2 *
-              (-1.6666667e-01f +
-               x2 * (8.3333333e-03f + x2 * (-1.9841270e-04f + x2 * 2.7557319e-06f)));
-  const float cs =
-      1.0f + x2 * (-0.5f + x2 * (4.1666668e-02f +
-                                 x2 * (-1.3888889e-03f + x2 * 2.4801587e-05f)));
-  const int q = (int)((long long)qn & 3);
-  return q == 0 ? mk<float>(cs, sn)
-                : q == 1 ? mk<float>(-sn, cs)
-                         : q == 2 ? mk<float>(-cs, -sn) : mk<float>(sn, -cs);
-}
-__device__ __forceinline__ cpx<double> phasor(double phi, double) {
-  double s, c;
-  sincos(phi, &s, &c);
-  return mk<double>(c, s);
-}
-
 // Column stage: for x-frequency column c and slices k0..k0+nb-1 build
 // S*g_k along the y-frequencies (zero outside the support), inverse
 // transform along y, crop rows padyl..padyl+Ny-1 (plus the fused

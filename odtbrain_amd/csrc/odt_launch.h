@@ -76,4 +76,19 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
                             int n, int logn, int dir, const cpx<R> *tw, int nb,
                             hipStream_t st);
 
+// Adjoint stages (odt_adjoint.hip).
+template <typename R>
+hipError_t launch_adj_rotate(int order, const cpx<R> *g, int nb,
+                             const double *params, cpx<R> *slab, int ny, int nz,
+                             int nx, hipStream_t st);
+template <typename R>
+hipError_t launch_scale_ends(R *data, long long nlines, int n, long long n0,
+                             long long s0, long long s1, long long es, double s,
+                             hipStream_t st);
+template <typename R>
+hipError_t launch_adj_fft(int stage, int n, const void *in, void *out,
+                          const void *tw, const void *aux, const double *prop_a,
+                          const double *weights, int a0, const Geom &g, int na,
+                          int ldt, hipStream_t st);
+
 }  // namespace odt

@@ -327,6 +327,55 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
   return ODT_OK;
 }
 
+// Adjoint of execute_t for the y-axis reconstruction (see odt_adjoint.hip).
+template <typename R>
+int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
+              hipStream_t st) {
+  const Geom &g = p->g;
+  cpx<R> *U1 = (cpx<R> *)(ws + p->off_F1);
+  cpx<R> *Sp = (cpx<R> *)(ws + p->off_S);
+  cpx<R> *Tp = (cpx<R> *)(ws + p->off_T);
+  cpx<R> *slab = (cpx<R> *)(ws + p->off_slab);
+  const cpx<R> *gout = (const cpx<R> *)d_gout;
+  cpx<R> *gu = (cpx<R> *)d_gusin;
+  const long long Nx = g.Nx, Ny = g.Ny, ln = g.ln;
+  const size_t per_angle = (size_t)Ny * ln * Nx;
+  for (int b0 = 0; b0 < p->A; b0 += p->NB) {
+    const int nb = std::min(p->NB, p->A - b0);
+    HIPCHK(odt::launch_adj_rotate<R>(p->order, gout, nb, p->params + 6 * (size_t)b0,
+                                     slab, g.Ny, g.ln, g.Nx, st),
+           "k_adj_rotate");
+    // Q^T = D Q D^-1 along x, then along z (real and imaginary lines)
+    R *sr = (R *)slab;
+    const long long lx = nb * Ny * ln * 2, lz = nb * Ny * Nx * 2;
+    HIPCHK(odt::launch_scale_ends<R>(sr, lx, g.Nx, 2, 1, Nx * 2, 2, 2.0, st), "ends x");
+    HIPCHK(odt::launch_iir<R>(sr, lx, g.Nx, 2, 1, Nx * 2, 2, p->order, st), "iir x");
+    HIPCHK(odt::launch_scale_ends<R>(sr, lx, g.Nx, 2, 1, Nx * 2, 2, 0.5, st), "ends x");
+    HIPCHK(odt::launch_scale_ends<R>(sr, lz, g.ln, Nx * 2, 1, ln * Nx * 2, Nx * 2, 2.0, st),
+           "ends z");
+    HIPCHK(odt::launch_iir<R>(sr, lz, g.ln, Nx * 2, 1, ln * Nx * 2, Nx * 2, p->order, st),
+           "iir z");
+    HIPCHK(odt::launch_scale_ends<R>(sr, lz, g.ln, Nx * 2, 1, ln * Nx * 2, Nx * 2, 0.5, st),
+           "ends z");
+    for (int c0 = b0; c0 < b0 + nb; c0 += p->NC) {
+      const int n = std::min(p->NC, b0 + nb - c0);
+      HIPCHK(odt::launch_adj_fft<R>(0, g.lNx, slab + (size_t)(c0 - b0) * per_angle, Tp,
+                                    p->twx, nullptr, nullptr, nullptr, 0, g, n, p->ldt, st),
+             "k_adj_row");
+      HIPCHK(odt::launch_adj_fft<R>(1, g.lNy, Tp, Sp, p->twy, nullptr, p->prop_a, nullptr,
+                                    0, g, n, p->ldt, st),
+             "k_adj_col");
+      HIPCHK(odt::launch_adj_fft<R>(2, g.lNy, Sp, U1, p->twy, p->P, nullptr, nullptr, 0, g,
+                                    n, g.Lx, st),
+             "k_adj_inv_col");
+      HIPCHK(odt::launch_adj_fft<R>(3, g.lNx, U1, gu, p->twx, nullptr, nullptr, p->weights,
+                                    c0, g, n, g.Lx, st),
+             "k_adj_inv_row");
+    }
+  }
+  return ODT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -502,6 +551,26 @@ int odt_plan_execute(odt_plan *p, const void *d_usin, void *d_out, void *ws,
 int odt_plan_destroy(odt_plan *p) {
   if (p) free_plan(p);
   return ODT_OK;
+}
+
+int odt_plan_adjoint(odt_plan *p, const void *d_gout, void *d_gusin, void *ws,
+                     size_t ws_bytes, void *stream) {
+  if (!p || !d_gout || !d_gusin) return fail(ODT_ERR_INVALID, "null argument");
+  if (p->kind != ODT_KIND_3D || p->onlyreal || !p->explicit_iir || p->order < 2 ||
+      (p->g.pad_mode != ODT_PAD_NONE && p->g.pad_mode != ODT_PAD_EDGE))
+    return fail(ODT_ERR_UNSUPPORTED,
+                "adjoint: y-axis 3D plans with complex output, the explicit "
+                "prefilter, spline order 2..5 and edge or no padding only");
+  auto pow2_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
+  if (!pow2_ok(p->g.lNx) || !pow2_ok(p->g.lNy))
+    return fail(ODT_ERR_UNSUPPORTED, "adjoint: padded sizes 64..512 (powers of two) only");
+  if (ws_bytes < p->ws_bytes || !ws)
+    return fail(ODT_ERR_WORKSPACE, "workspace too small: need " +
+                                       std::to_string(p->ws_bytes) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p->precision == ODT_FP64
+             ? adjoint_t<double>(p, d_gout, d_gusin, (char *)ws, st)
+             : adjoint_t<float>(p, d_gout, d_gusin, (char *)ws, st);
 }
 
 struct odt_batch {

@@ -68,14 +68,15 @@ def random_ellipsoids(count, semi=(20, 60), dn=(0.005, 0.02), spread=30.0,
 
 
 def ellipsoid_sinogram_3d(N=256, A=360, count=3, res=6.0, phase_noise=0.02,
-                          seed=1, dtype=np.complex64):
+                          seed=1, dtype=np.complex64, semi=(20, 60),
+                          spread=30.0):
     """Rytov sinogram uB = i*phase of seeded overlapping ellipsoids.
 
     phase = k0 * sum_j dn_j * chord_j + N(0, phase_noise) (radians);
     returns (uB (A, N, N), angles (A,)).
     """
     rng = np.random.default_rng(seed)
-    ells = random_ellipsoids(count, seed=seed)
+    ells = random_ellipsoids(count, semi=semi, spread=spread, seed=seed)
     angles = np.linspace(0, 2 * np.pi, A, endpoint=False)
     k0 = 2 * np.pi / res
     out = np.empty((A, N, N), dtype=dtype)
