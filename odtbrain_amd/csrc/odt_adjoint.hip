@@ -32,44 +32,109 @@ namespace odt {
 
 extern __shared__ __align__(16) unsigned char odt_smem[];
 
-template <typename R>
-__global__ void k_fill_zero(R *p, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = R(0);
-}
-
-// R_a^T: every output voxel (z, y, x) of every angle adds g * w to the taps
-// it reads in the forward gather (k_rotate_lds / scipy's rotate).
-template <typename R, int ORDER>
+// R_a^T as a gather: the coefficient (a, y, jz, jx) sums w * g over the
+// voxels whose forward taps (k_rotate_lds / scipy's rotate: same float64
+// coordinates, strict bounds test, tap start and mirrored taps) include it,
+// directly or through the whole-sample mirror (virtual taps -j and
+// 2n - 2 - j).  Those voxels lie in the inverse-rotated square of half
+// width (ORDER + 1) / 2 about the tap; the candidates of its bounding box
+// are tested exactly.  The rotation is about y, so one candidate search
+// serves YC planes.  Block: a 16 x 16 coefficient tile of one angle and
+// YC planes; every coefficient is written (no atomics, no clearing).
+template <typename R, int ORDER, int YC>
 __global__ void __launch_bounds__(256)
     k_adj_rotate(const cpx<R> *__restrict__ g, const double *__restrict__ params,
                  cpx<R> *__restrict__ slab, int ny, int nz, int nx) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ix >= nz * nx) return;
-  const int z = ix / nx, x = ix - z * nx;
-  const int y = blockIdx.y, a = blockIdx.z;
+  const int ntx = (nx + 15) >> 4;
+  const int tz = blockIdx.x / ntx, tx = blockIdx.x - tz * ntx;
+  const int jz = tz * 16 + (threadIdx.x >> 4);
+  const int jx = tx * 16 + (threadIdx.x & 15);
+  if (jz >= nz || jx >= nx) return;
+  const int a = blockIdx.y, y0 = blockIdx.z * YC;
+  const int yc = min(YC, ny - y0);
   const double *p = params + 6 * (size_t)a;
-  const double cz = affine2((double)z, (double)x, p[0], p[1], p[4]);
-  const double cx = affine2((double)z, (double)x, p[2], p[3], p[5]);
-  if (cz < 0.0 || cz > (double)(nz - 1) || cx < 0.0 || cx > (double)(nx - 1))
-    return;
-  double wz[ORDER + 1], wx[ORDER + 1];
-  spline_weights<ORDER, double>(cz, wz);
-  spline_weights<ORDER, double>(cx, wx);
-  const int sz = spline_start<ORDER>(cz), sx = spline_start<ORDER>(cx);
-  const cpx<R> gv = g[((size_t)z * ny + y) * nx + x];
-  R *pl = reinterpret_cast<R *>(slab + ((size_t)a * ny + y) * nz * nx);
+  const double m00 = p[0], m01 = p[1], m10 = p[2], m11 = p[3];
+  const double oz = p[4], ox = p[5];
+  constexpr double h = 0.5 * (ORDER + 1);
+  constexpr int K = ORDER / 2 + 1;  // taps reach at most K past an edge
+  const double hz = h * (fabs(m00) + fabs(m10)) + 1e-7;
+  const double hx = h * (fabs(m01) + fabs(m11)) + 1e-7;
+  const int nzc = (int)floor(2.0 * hz) + 1, nxc = (int)floor(2.0 * hx) + 1;
+  constexpr int NONE = -(1 << 30);
+  const int vz1 = (jz >= 1 && jz <= K)                    ? -jz
+                  : (jz <= nz - 2 && jz >= nz - 1 - K) ? 2 * nz - 2 - jz
+                                                        : NONE;
+  const int vx1 = (jx >= 1 && jx <= K)                    ? -jx
+                  : (jx <= nx - 2 && jx >= nx - 1 - K) ? 2 * nx - 2 - jx
+                                                        : NONE;
+  cpx<R> acc[YC];
 #pragma unroll
-  for (int i = 0; i <= ORDER; ++i) {
-    const int zr = mirror_idx(sz + i, nz);
+  for (int k = 0; k < YC; ++k) acc[k] = mk<R>(0, 0);
+  for (int iv = 0; iv < 4; ++iv) {
+    const int vz = (iv & 1) ? vz1 : jz, vx = (iv & 2) ? vx1 : jx;
+    if (vz == NONE || vx == NONE) continue;
+    const double dz = (double)vz - oz, dx = (double)vx - ox;
+    const int zlo = (int)ceil(m00 * dz + m10 * dx - hz);
+    const int xlo = (int)ceil(m01 * dz + m11 * dx - hx);
+    for (int iz = 0; iz < nzc; ++iz) {
+      const int z = zlo + iz;
+      if (z < 0 || z >= nz) continue;
+      for (int ix = 0; ix < nxc; ++ix) {
+        const int x = xlo + ix;
+        if (x < 0 || x >= nx) continue;
+        const double cz = affine2((double)z, (double)x, m00, m01, oz);
+        const double cx = affine2((double)z, (double)x, m10, m11, ox);
+        if (cz < 0.0 || cz > (double)(nz - 1) || cx < 0.0 ||
+            cx > (double)(nx - 1))
+          continue;
+        const unsigned tzi = (unsigned)(vz - spline_start<ORDER>(cz));
+        const unsigned txi = (unsigned)(vx - spline_start<ORDER>(cx));
+        if (tzi > (unsigned)ORDER || txi > (unsigned)ORDER) continue;
+        const R w = (R)(bspline_val<ORDER>(cz - (double)vz) *
+                        bspline_val<ORDER>(cx - (double)vx));
+        const cpx<R> *gp = g + ((size_t)z * ny + y0) * nx + x;
 #pragma unroll
-    for (int j = 0; j <= ORDER; ++j) {
-      const int xc = mirror_idx(sx + j, nx);
-      const double w = wz[i] * wx[j];
-      R *dst = pl + 2 * ((size_t)zr * nx + xc);
-      unsafeAtomicAdd(dst, (R)(w * (double)gv.re));
-      unsafeAtomicAdd(dst + 1, (R)(w * (double)gv.im));
+        for (int k = 0; k < YC; ++k)
+          if (k < yc) acc[k] = acc[k] + w * gp[(size_t)k * nx];
+      }
     }
+  }
+  cpx<R> *dst = slab + (((size_t)a * ny + y0) * nz + jz) * nx + jx;
+#pragma unroll
+  for (int k = 0; k < YC; ++k)
+    if (k < yc) dst[(size_t)k * nz * nx] = acc[k];
+}
+
+// Q^T = D Q D^{-1} along x (D = diag(1/2, 1, ..., 1, 1/2), Q the mirror
+// B-spline prefilter of k_iir) for contiguous complex rows of length n:
+// a block stages rb rows in LDS (row stride 2n + 2 values, so the 2 rb
+// lines of a wave fall in distinct banks) and runs one line per lane.
+template <typename R>
+__global__ void __launch_bounds__(64)
+    k_adj_iir_rows(R *__restrict__ data, long long nrows, int n, int rb,
+                   int npoles, double z1, double z2, double gain) {
+  R *buf = reinterpret_cast<R *>(odt_smem);
+  const int ld = 2 * n + 2, w = 2 * n;
+  const long long r0 = (long long)blockIdx.x * rb;
+  const int nr = (int)min((long long)rb, nrows - r0);
+  R *src = data + r0 * w;
+  for (int i = threadIdx.x; i < nr * w; i += 64) {
+    const int r = i / w;
+    buf[r * ld + (i - r * w)] = src[i];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < 2 * nr && n >= 2) {
+    R *q = buf + (threadIdx.x >> 1) * ld + (threadIdx.x & 1);
+    q[0] *= R(2);
+    q[2 * (n - 1)] *= R(2);
+    iir_line<R>(q, n, 2, npoles, z1, z2, gain);
+    q[0] *= R(0.5);
+    q[2 * (n - 1)] *= R(0.5);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nr * w; i += 64) {
+    const int r = i / w;
+    src[i] = buf[r * ld + (i - r * w)];
   }
 }
 
@@ -281,8 +346,9 @@ template <typename R, int ORDER>
 static hipError_t adj_rotate_launch(const cpx<R> *g, int nb, const double *params,
                                     cpx<R> *slab, int ny, int nz, int nx,
                                     hipStream_t st) {
-  dim3 grid((nz * nx + 255) / 256, ny, nb);
-  k_adj_rotate<R, ORDER><<<grid, 256, 0, st>>>(g, params, slab, ny, nz, nx);
+  constexpr int YC = sizeof(R) == 4 ? 16 : 8;
+  dim3 grid(((nz + 15) / 16) * ((nx + 15) / 16), nb, (ny + YC - 1) / YC);
+  k_adj_rotate<R, ORDER, YC><<<grid, 256, 0, st>>>(g, params, slab, ny, nz, nx);
   return hipGetLastError();
 }
 
@@ -290,10 +356,7 @@ template <typename R>
 hipError_t launch_adj_rotate(int order, const cpx<R> *g, int nb,
                              const double *params, cpx<R> *slab, int ny, int nz,
                              int nx, hipStream_t st) {
-  const size_t n = (size_t)nb * ny * nz * nx * 2;
-  k_fill_zero<R><<<(unsigned)((n + 255) / 256), 256, 0, st>>>((R *)slab, n);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (nz < 8 || nx < 8) return hipErrorInvalidValue;
   switch (order) {
     case 2: return adj_rotate_launch<R, 2>(g, nb, params, slab, ny, nz, nx, st);
     case 3: return adj_rotate_launch<R, 3>(g, nb, params, slab, ny, nz, nx, st);
@@ -301,6 +364,23 @@ hipError_t launch_adj_rotate(int order, const cpx<R> *g, int nb,
     case 5: return adj_rotate_launch<R, 5>(g, nb, params, slab, ny, nz, nx, st);
   }
   return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_adj_prefilter_rows(cpx<R> *rows, long long nrows, int n,
+                                     int order, hipStream_t st) {
+  int np = 0;
+  double z1 = 0, z2 = 0, gain = 1;
+  if (!iir_poles(order, &np, &z1, &z2, &gain)) return hipErrorInvalidValue;
+  if (nrows <= 0) return hipSuccess;
+  const size_t row_bytes = (size_t)(2 * n + 2) * sizeof(R);
+  const int rb = (int)std::max<size_t>(1, std::min<size_t>(32, 65536 / row_bytes));
+  const size_t lds = rb * row_bytes;
+  if (lds > 65536) return hipErrorInvalidValue;
+  const long long blocks = (nrows + rb - 1) / rb;
+  k_adj_iir_rows<R><<<(unsigned)blocks, 64, lds, st>>>(
+      (R *)rows, nrows, n, rb, np, z1, z2, gain);
+  return hipGetLastError();
 }
 
 template <typename R>
@@ -362,6 +442,8 @@ hipError_t launch_adj_fft(int stage, int n, const void *in, void *out,
 }
 
 #define ODT_ADJ_INST(R)                                                         \
+  template hipError_t launch_adj_prefilter_rows<R>(cpx<R> *, long long, int,  \
+                                                   int, hipStream_t);          \
   template hipError_t launch_adj_rotate<R>(int, const cpx<R> *, int,            \
                                            const double *, cpx<R> *, int, int,  \
                                            int, hipStream_t);                   \

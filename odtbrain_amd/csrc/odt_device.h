@@ -201,6 +201,81 @@ __device__ __forceinline__ void spline_weights(double c, W *w) {
   }
 }
 
+// Mirror-boundary B-spline prefilter of one line (elements p[i * es],
+// n >= 2): gain, then per pole the causal pass with the exact mirror
+// initial value (sum over the whole line) and the anti-causal pass
+// (Unser/Thevenaz recursive filter), in float64 with values stored as R.
+template <typename R>
+__device__ __forceinline__ void iir_line(R *p, int n, long long es, int npoles,
+                                         double z1, double z2, double gain) {
+  for (int i = 0; i < n; ++i) p[i * es] = (R)((double)p[i * es] * gain);
+  for (int pi = 0; pi < npoles; ++pi) {
+    const double z = pi == 0 ? z1 : z2;
+    const double zn1 = pow(z, (double)(n - 1));
+    double c0 = (double)p[0] + zn1 * (double)p[(n - 1) * es];
+    double zi = z;
+    for (int i = 1; i < n - 1; ++i) {
+      c0 += zi * ((double)p[i * es] + zn1 * (double)p[(n - 1 - i) * es]);
+      zi *= z;
+    }
+    double prev = c0 / (1.0 - zn1 * zn1);
+    p[0] = (R)prev;
+    for (int i = 1; i < n; ++i) {
+      prev = (double)p[i * es] + z * prev;
+      p[i * es] = (R)prev;
+    }
+    double next = (z * (double)p[(n - 2) * es] + prev) * z / (z * z - 1.0);
+    p[(n - 1) * es] = (R)next;
+    for (int i = n - 2; i >= 0; --i) {
+      next = z * (next - (double)p[i * es]);
+      p[i * es] = (R)next;
+    }
+  }
+}
+
+// Centred B-spline of an order (2..5) at distance d: the weight the
+// forward interpolation gives the tap at distance d (zero outside
+// |d| < (ORDER + 1) / 2).
+template <int ORDER>
+__device__ __forceinline__ double bspline_val(double d) {
+  d = fabs(d);
+  if (ORDER == 2) {
+    if (d < 0.5) return 0.75 - d * d;
+    if (d < 1.5) return 0.5 * (1.5 - d) * (1.5 - d);
+    return 0.0;
+  } else if (ORDER == 3) {
+    if (d < 1.0) return (d * d * (d - 2.0) * 3.0 + 4.0) / 6.0;
+    if (d < 2.0) return (2.0 - d) * (2.0 - d) * (2.0 - d) / 6.0;
+    return 0.0;
+  } else if (ORDER == 4) {
+    if (d < 0.5) {
+      const double d2 = d * d;
+      return d2 * (d2 * 0.25 - 0.625) + 115.0 / 192.0;
+    }
+    if (d < 1.5)
+      return (55.0 + d * (20.0 + d * (-120.0 + d * (80.0 - 16.0 * d)))) /
+             96.0;
+    if (d < 2.5) {
+      const double e = 2.5 - d;
+      return e * e * e * e / 24.0;
+    }
+    return 0.0;
+  } else {
+    if (d < 1.0) {
+      const double d2 = d * d;
+      return d2 * (d2 * (0.25 - d / 12.0) - 0.5) + 0.55;
+    }
+    if (d < 2.0)
+      return d * (d * (d * (d * (d / 24.0 - 0.375) + 1.25) - 1.75) + 0.625) +
+             0.425;
+    if (d < 3.0) {
+      const double e = 3.0 - d;
+      return e * e * e * e * e / 120.0;
+    }
+    return 0.0;
+  }
+}
+
 // Same weights from the fractional offset t = c - floor(c) (odd orders) or
 // t = c - floor(c + 0.5) (even orders), evaluated in W: the float32 path
 // forms t exactly in float64 and only the polynomials run in float32.

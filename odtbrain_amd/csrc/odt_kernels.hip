@@ -1127,30 +1127,8 @@ __global__ void k_iir(R *data, long long nlines, int n, long long n0,
                       double z1, double z2, double gain) {
   const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nlines || n < 2) return;
-  R *p = data + (l / n0) * s1 + (l % n0) * s0;
-  for (int i = 0; i < n; ++i) p[i * es] = (R)((double)p[i * es] * gain);
-  for (int pi = 0; pi < npoles; ++pi) {
-    const double z = pi == 0 ? z1 : z2;
-    const double zn1 = pow(z, (double)(n - 1));
-    double c0 = (double)p[0] + zn1 * (double)p[(n - 1) * es];
-    double zi = z;
-    for (int i = 1; i < n - 1; ++i) {
-      c0 += zi * ((double)p[i * es] + zn1 * (double)p[(n - 1 - i) * es]);
-      zi *= z;
-    }
-    double prev = c0 / (1.0 - zn1 * zn1);
-    p[0] = (R)prev;
-    for (int i = 1; i < n; ++i) {
-      prev = (double)p[i * es] + z * prev;
-      p[i * es] = (R)prev;
-    }
-    double next = (z * (double)p[(n - 2) * es] + prev) * z / (z * z - 1.0);
-    p[(n - 1) * es] = (R)next;
-    for (int i = n - 2; i >= 0; --i) {
-      next = z * (next - (double)p[i * es]);
-      p[i * es] = (R)next;
-    }
-  }
+  iir_line<R>(data + (l / n0) * s1 + (l % n0) * s0, n, es, npoles, z1, z2,
+              gain);
 }
 
 // batched 1D transform of contiguous vectors
@@ -1494,29 +1472,36 @@ hipError_t launch_seq_rotate_sum(int order, V *slab, int nb,
   return hipGetLastError();
 }
 
+bool iir_poles(int order, int *np, double *z1, double *z2, double *gain) {
+  *np = 1;
+  *z2 = 0;
+  switch (order) {
+    case 2: *z1 = sqrt(8.0) - 3.0; break;
+    case 3: *z1 = sqrt(3.0) - 2.0; break;
+    case 4:
+      *np = 2;
+      *z1 = sqrt(664.0 - sqrt(438976.0)) + sqrt(304.0) - 19.0;
+      *z2 = sqrt(664.0 + sqrt(438976.0)) - sqrt(304.0) - 19.0;
+      break;
+    case 5:
+      *np = 2;
+      *z1 = sqrt(67.5 - sqrt(4436.25)) + sqrt(26.25) - 6.5;
+      *z2 = sqrt(67.5 + sqrt(4436.25)) - sqrt(26.25) - 6.5;
+      break;
+    default: return false;
+  }
+  *gain = (1.0 - *z1) * (1.0 - 1.0 / *z1);
+  if (*np == 2) *gain *= (1.0 - *z2) * (1.0 - 1.0 / *z2);
+  return true;
+}
+
 template <typename R>
 hipError_t launch_iir(R *data, long long nlines, int n, long long n0,
                       long long s0, long long s1, long long es, int order,
                       hipStream_t st) {
-  double z1 = 0, z2 = 0;
-  int np = 1;
-  switch (order) {
-    case 2: z1 = sqrt(8.0) - 3.0; break;
-    case 3: z1 = sqrt(3.0) - 2.0; break;
-    case 4:
-      np = 2;
-      z1 = sqrt(664.0 - sqrt(438976.0)) + sqrt(304.0) - 19.0;
-      z2 = sqrt(664.0 + sqrt(438976.0)) - sqrt(304.0) - 19.0;
-      break;
-    case 5:
-      np = 2;
-      z1 = sqrt(67.5 - sqrt(4436.25)) + sqrt(26.25) - 6.5;
-      z2 = sqrt(67.5 + sqrt(4436.25)) - sqrt(26.25) - 6.5;
-      break;
-    default: return hipErrorInvalidValue;
-  }
-  double gain = (1.0 - z1) * (1.0 - 1.0 / z1);
-  if (np == 2) gain *= (1.0 - z2) * (1.0 - 1.0 / z2);
+  double z1 = 0, z2 = 0, gain = 1;
+  int np = 0;
+  if (!iir_poles(order, &np, &z1, &z2, &gain)) return hipErrorInvalidValue;
   if (nlines <= 0) return hipSuccess;
   const long long blocks = (nlines + 255) / 256;
   k_iir<R><<<(unsigned)blocks, 256, 0, st>>>(data, nlines, n, n0, s0, s1, es,

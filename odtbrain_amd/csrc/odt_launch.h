@@ -67,6 +67,9 @@ hipError_t launch_seq_rotate_sum(int order, V *slab, int nb,
                                  const double *params, V *out, V *comp,
                                  int ny, int nz, int nx, int first, int last,
                                  hipStream_t st);
+// Poles and gain of the mirror-boundary B-spline prefilter of an order
+// (2..5); false for other orders.
+bool iir_poles(int order, int *np, double *z1, double *z2, double *gain);
 template <typename R>
 hipError_t launch_iir(R *data, long long nlines, int n, long long n0,
                       long long s0, long long s1, long long es, int order,
@@ -77,6 +80,9 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
                             hipStream_t st);
 
 // Adjoint stages (odt_adjoint.hip).
+template <typename R>
+hipError_t launch_adj_prefilter_rows(cpx<R> *rows, long long nrows, int n,
+                                     int order, hipStream_t st);
 template <typename R>
 hipError_t launch_adj_rotate(int order, const cpx<R> *g, int nb,
                              const double *params, cpx<R> *slab, int ny, int nz,

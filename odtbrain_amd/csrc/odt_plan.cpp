@@ -347,10 +347,9 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
            "k_adj_rotate");
     // Q^T = D Q D^-1 along x, then along z (real and imaginary lines)
     R *sr = (R *)slab;
-    const long long lx = nb * Ny * ln * 2, lz = nb * Ny * Nx * 2;
-    HIPCHK(odt::launch_scale_ends<R>(sr, lx, g.Nx, 2, 1, Nx * 2, 2, 2.0, st), "ends x");
-    HIPCHK(odt::launch_iir<R>(sr, lx, g.Nx, 2, 1, Nx * 2, 2, p->order, st), "iir x");
-    HIPCHK(odt::launch_scale_ends<R>(sr, lx, g.Nx, 2, 1, Nx * 2, 2, 0.5, st), "ends x");
+    const long long lz = nb * Ny * Nx * 2;
+    HIPCHK(odt::launch_adj_prefilter_rows<R>(slab, nb * Ny * ln, g.Nx, p->order, st),
+           "k_adj_iir_rows");
     HIPCHK(odt::launch_scale_ends<R>(sr, lz, g.ln, Nx * 2, 1, ln * Nx * 2, Nx * 2, 2.0, st),
            "ends z");
     HIPCHK(odt::launch_iir<R>(sr, lz, g.ln, Nx * 2, 1, ln * Nx * 2, Nx * 2, p->order, st),
@@ -564,6 +563,8 @@ int odt_plan_adjoint(odt_plan *p, const void *d_gout, void *d_gusin, void *ws,
   auto pow2_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
   if (!pow2_ok(p->g.lNx) || !pow2_ok(p->g.lNy))
     return fail(ODT_ERR_UNSUPPORTED, "adjoint: padded sizes 64..512 (powers of two) only");
+  if (p->g.Nx < 8)
+    return fail(ODT_ERR_UNSUPPORTED, "adjoint: detector width of at least 8 pixels");
   if (ws_bytes < p->ws_bytes || !ws)
     return fail(ODT_ERR_WORKSPACE, "workspace too small: need " +
                                        std::to_string(p->ws_bytes) + " bytes");
