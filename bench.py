@@ -326,6 +326,25 @@ def main():
     log(f"bench: fp32 {t_step * 1e3:.1f} ms/step, rotate {rot_stage_ms:.2f} ms "
         f"in {launches} launches, plan total {np.mean(tot):.1f} ms")
 
+    # Epilogue after the path: odt_to_ri of the 256^3 complex64 volume
+    # (odtbrain/_translate_ri.py:5-57), one elementwise HBM stream, timed
+    # with events on the stream it runs on; 16 B per voxel (read + write).
+    ri = odtbrain_amd.odt_to_ri(out32, RES, NM)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), \
+        torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(20):
+        ri = odtbrain_amd.odt_to_ri(out32, RES, NM)
+    ev1.record()
+    torch.cuda.synchronize()
+    ri_ms = ev0.elapsed_time(ev1) / 20
+    ri_gbs = 16 * V / (ri_ms * 1e-3) / 1e9
+    epilogue = {"function": "odt_to_ri", "kernel": "k_odt_to_ri_c",
+                "ms": ri_ms, "achieved_GBps": ri_gbs,
+                "frac": ri_gbs / HBM_PEAK_GBS,
+                "algorithmic_bytes": 16 * V}
+    del ri
+
     fp64 = None
     if not args.no_fp64:
         log("bench: fp64 path")
@@ -380,6 +399,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,
+        "epilogue": epilogue,
     }
     print(json.dumps(line), flush=True)
 

@@ -177,6 +177,20 @@ int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
                           const double *params, void *d_out, int accumulate,
                           void *stream);
 
+/* Object function to refractive index, elementwise over n values
+ * (odtbrain/_translate_ri.py:5-57, odt_to_ri): ri = nm * sqrt(f / km^2 + 1)
+ * with km = 2 pi nm / res (km^2 formed in float64 on the host), the
+ * principal root (non-negative real part) for complex f.  is_complex: 1 for
+ * interleaved complex f and ri, 0 for real (a negative radicand gives NaN,
+ * as numpy's real sqrt does).  In place allowed; asynchronous. */
+int odt_odt_to_ri(int precision, int is_complex, int64_t n, double res,
+                  double nm, const void *d_f, void *d_ri, void *stream);
+
+/* Optical projection tomography variant (_translate_ri.py:60-90,
+ * opt_to_ri): ri = nm + f / (2 pi) * res, elementwise as above. */
+int odt_opt_to_ri(int precision, int is_complex, int64_t n, double res,
+                  double nm, const void *d_f, void *d_ri, void *stream);
+
 const char *odt_last_error(void);
 int odt_version(void);
 

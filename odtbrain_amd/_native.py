@@ -21,6 +21,7 @@ EXPORTED = (
     "odt_fft_c2c", "odt_spline_prefilter", "odt_rotate_accumulate",
     "odt_plan_adjoint",
     "odt_batch_create", "odt_batch_launch", "odt_batch_destroy",
+    "odt_odt_to_ri", "odt_opt_to_ri",
     "odt_last_error", "odt_version",
 )
 
@@ -92,6 +93,10 @@ def lib():
                                    ctypes.c_size_t, vp, vpp]
     L.odt_batch_launch.argtypes = [vp, vp]
     L.odt_batch_destroy.argtypes = [vp]
+    for name in ("odt_odt_to_ri", "odt_opt_to_ri"):
+        getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_int, i64,
+                                     ctypes.c_double, ctypes.c_double, vp, vp,
+                                     vp]
     L.odt_last_error.restype = ctypes.c_char_p
     L.odt_last_error.argtypes = []
     L.odt_version.argtypes = []

@@ -79,6 +79,15 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
                             int n, int logn, int dir, const cpx<R> *tw, int nb,
                             hipStream_t st);
 
+// Object function to refractive index (odt_translate.hip); n elements
+// (complex elements when is_complex), in place allowed.
+template <typename R>
+hipError_t launch_odt_to_ri(int is_complex, long long n, double km2, double nm,
+                            const void *f, void *ri, hipStream_t st);
+template <typename R>
+hipError_t launch_opt_to_ri(int is_complex, long long n, double res, double nm,
+                            const void *f, void *ri, hipStream_t st);
+
 // Adjoint stages (odt_adjoint.hip).
 template <typename R>
 hipError_t launch_adj_prefilter_rows(cpx<R> *rows, long long nrows, int n,

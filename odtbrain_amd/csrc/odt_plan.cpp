@@ -699,6 +699,35 @@ int odt_fft_c2c(int precision, int64_t batch, int32_t n, int32_t direction,
   return ODT_OK;
 }
 
+int odt_odt_to_ri(int precision, int is_complex, int64_t n, double res, double nm,
+                  const void *d_f, void *d_ri, void *stream) {
+  if (precision != ODT_FP32 && precision != ODT_FP64)
+    return fail(ODT_ERR_INVALID, "precision");
+  if (n < 0 || (n > 0 && (!d_f || !d_ri)) || !(res != 0.0))
+    return fail(ODT_ERR_INVALID, "bad arguments");
+  const double km = (2 * M_PI * nm) / res;
+  const double km2 = km * km;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = precision == ODT_FP64
+                     ? odt::launch_odt_to_ri<double>(is_complex, n, km2, nm, d_f, d_ri, st)
+                     : odt::launch_odt_to_ri<float>(is_complex, n, km2, nm, d_f, d_ri, st);
+  if (e != hipSuccess) return hip_fail(e, "k_odt_to_ri");
+  return ODT_OK;
+}
+
+int odt_opt_to_ri(int precision, int is_complex, int64_t n, double res, double nm,
+                  const void *d_f, void *d_ri, void *stream) {
+  if (precision != ODT_FP32 && precision != ODT_FP64)
+    return fail(ODT_ERR_INVALID, "precision");
+  if (n < 0 || (n > 0 && (!d_f || !d_ri))) return fail(ODT_ERR_INVALID, "bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = precision == ODT_FP64
+                     ? odt::launch_opt_to_ri<double>(is_complex, n, res, nm, d_f, d_ri, st)
+                     : odt::launch_opt_to_ri<float>(is_complex, n, res, nm, d_f, d_ri, st);
+  if (e != hipSuccess) return hip_fail(e, "k_opt_to_ri");
+  return ODT_OK;
+}
+
 int odt_spline_prefilter(int precision, int order, int64_t nlines, int32_t n,
                          int64_t line_stride, int64_t elem_stride, void *d_data,
                          void *stream) {

@@ -180,3 +180,34 @@ def test_oracle_planted_mistake_is_caught():
     ref = ref[ref.size // 2:].reshape(good.shape)
     assert np.allclose(good, ref)
     assert not np.allclose(bad, ref)
+
+
+# --------------------------------------------------------------------------
+# odt_to_ri / opt_to_ri (odtbrain/_translate_ri.py)
+# --------------------------------------------------------------------------
+def test_translate_oracle_fixtures():
+    from oracle import translate as T
+    f, res, nm = T.processing_test_data()
+    for name, fn in (("processing__test_odt_to_ri", T.odt_to_ri),
+                     ("processing__test_opt_to_ri", T.opt_to_ri)):
+        ri = fn(f, res, nm)
+        assert np.allclose(np.array(ri).flatten().view(float),
+                           load_golden(name))
+        assert np.allclose(fn(f[0], res, nm), ri[0])
+
+
+def test_odt_to_ri_oracle_inverts_the_object_function():
+    """f = km^2 ((n/nm)^2 - 1) maps back to n for any n with positive real
+    part (a dropped km^2 or nm, or the other root, fails this)."""
+    from oracle import translate as T
+    rng = np.random.default_rng(3)
+    res, nm = 6.5, 1.333
+    km = 2 * np.pi * nm / res
+    n = rng.uniform(1.0, 1.6, 500) + 1j * rng.uniform(-0.2, 0.2, 500)
+    f = km ** 2 * ((n / nm) ** 2 - 1)
+    assert np.allclose(T.odt_to_ri(f, res, nm), n, rtol=1e-13, atol=0)
+    # opt_to_ri is affine with slope res / (2 pi) and value nm at 0
+    g = rng.standard_normal(50)
+    assert np.allclose(T.opt_to_ri(g, res, nm),
+                       nm + g * res / (2 * np.pi))
+    assert T.opt_to_ri(np.zeros(1), res, nm)[0] == nm
