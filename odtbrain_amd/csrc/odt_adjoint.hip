@@ -105,6 +105,106 @@ __global__ void __launch_bounds__(256)
     if (k < yc) dst[(size_t)k * nz * nx] = acc[k];
 }
 
+// Q^T = D Q D^{-1} along one axis of complex lines, out of place, with Q
+// the mirror-boundary B-spline prefilter written as a convolution: the
+// recursive filter with its whole-line mirror initialisation equals the
+// two-sided impulse response h of the filter applied to the infinitely
+// mirrored line (h decays as z^|k|; the host truncates it where it falls
+// below 1e-18 of h[0]).  Line l starts at (l / n0) * s1 + (l % n0) * s0,
+// element i at + i * es (complex units); consecutive threads take
+// consecutive lines, each thread kFirOB consecutive outputs, streaming the
+// kFirOB + 2K inputs once.  hz holds h[k] at k + K + kFirOB, zero-padded
+// to 2K + 2 kFirOB entries, so every (output, input) pair indexes inside;
+// its index is uniform, so the taps come through the scalar cache.
+constexpr int kFirOB = 8;
+constexpr int adj_fir_pad_c = kFirOB;
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_adj_fir(const cpx<R> *__restrict__ in, cpx<R> *__restrict__ out,
+              long long nlines, int n, long long n0, long long s0, long long s1,
+              long long es, const R *__restrict__ hz, int K, int nzb) {
+  // Blocks are dealt round-robin to the 8 XCDs: all nzb output blocks of a
+  // group of 256 lines go to one XCD, so their overlapping input windows
+  // meet in its L2.
+  const int bid = blockIdx.x, xcd = bid & 7, k = bid >> 3;
+  const long long grp = (long long)(k / nzb) * 8 + xcd;
+  const int i0 = (k % nzb) * kFirOB;
+  const long long l = grp * blockDim.x + threadIdx.x;
+  if (l >= nlines) return;
+  const long long base = (l / n0) * s1 + (l % n0) * s0;
+  const cpx<R> *src = in + base;
+  cpx<R> acc[kFirOB];
+#pragma unroll
+  for (int o = 0; o < kFirOB; ++o) acc[o] = mk<R>(0, 0);
+  const int jn = kFirOB + 2 * K;
+  // inputs in groups of 8 loads issued together, then their updates
+  for (int j0 = 0; j0 < jn; j0 += 8) {
+    cpx<R> v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      v[u] = mk<R>(0, 0);
+      if (j0 + u < jn) {
+        const int m = mirror_idx(i0 - K + j0 + u, n);
+        v[u] = src[(long long)m * es];
+        if (m == 0 || m == n - 1) v[u] = mk<R>(R(2) * v[u].re, R(2) * v[u].im);  // D^-1
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (j0 + u < jn) {
+        const R *w = hz + (2 * K + kFirOB - (j0 + u));
+#pragma unroll
+        for (int o = 0; o < kFirOB; ++o) acc[o] = acc[o] + w[o] * v[u];
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < kFirOB; ++o) {
+    const int i = i0 + o;
+    if (i < n) {
+      cpx<R> r = acc[o];
+      if (i == 0 || i == n - 1) r = mk<R>(R(0.5) * r.re, R(0.5) * r.im);  // D
+      out[base + (long long)i * es] = r;
+    }
+  }
+}
+
+// The same convolution along contiguous rows of length n: each wave
+// copies one row, mirror-extended by K on both sides and scaled by D^-1 at
+// its ends, into LDS and its lanes produce consecutive outputs from there.
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_adj_fir_rows(const cpx<R> *__restrict__ in, cpx<R> *__restrict__ out,
+                   long long nrows, int n, const R *__restrict__ hz, int K) {
+  cpx<R> *ext = reinterpret_cast<cpx<R> *>(odt_smem);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ne = n + 2 * K;
+  cpx<R> *e = ext + (size_t)wave * ne;
+  const long long row = (long long)blockIdx.x * 4 + wave;
+  const bool live = row < nrows;
+  if (live) {
+    const cpx<R> *src = in + row * n;
+    for (int i = lane; i < ne; i += 64) {
+      const int m = mirror_idx(i - K, n);
+      cpx<R> v = src[m];
+      if (m == 0 || m == n - 1) v = mk<R>(R(2) * v.re, R(2) * v.im);  // D^-1
+      e[i] = v;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const R *h = hz + adj_fir_pad_c;  // h[k] at h[k + K]
+  for (int x = lane; x < n; x += 64) {
+    cpx<R> acc = mk<R>(0, 0);
+    const cpx<R> *ex = e + x + 2 * K;  // ex[-k - K] = ext[x + K - k]
+#pragma unroll 8
+    for (int k = 0; k <= 2 * K; ++k) acc = acc + h[k] * ex[-k];
+    if (x == 0 || x == n - 1) acc = mk<R>(R(0.5) * acc.re, R(0.5) * acc.im);  // D
+    out[row * n + x] = acc;
+  }
+}
+
 // Q^T = D Q D^{-1} along x (D = diag(1/2, 1, ..., 1, 1/2), Q the mirror
 // B-spline prefilter of k_iir) for contiguous complex rows of length n:
 // a block stages rb rows in LDS (row stride 2n + 2 values, so the 2 rb
@@ -367,6 +467,34 @@ hipError_t launch_adj_rotate(int order, const cpx<R> *g, int nb,
 }
 
 template <typename R>
+hipError_t launch_adj_fir(const cpx<R> *in, cpx<R> *out, long long nlines, int n,
+                          long long n0, long long s0, long long s1, long long es,
+                          const R *hz, int K, hipStream_t st) {
+  if (nlines <= 0 || n < 1) return hipSuccess;
+  if (n0 < 1) return hipErrorInvalidValue;
+  const long long ngrp = (nlines + 255) / 256;
+  const int nzb = (n + kFirOB - 1) / kFirOB;
+  const long long blocks = ((ngrp + 7) / 8) * 8 * nzb;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  k_adj_fir<R><<<(unsigned)blocks, 256, 0, st>>>(in, out, nlines, n, n0, s0, s1, es, hz,
+                                                 K, nzb);
+  return hipGetLastError();
+}
+
+int adj_fir_pad() { return adj_fir_pad_c; }
+
+template <typename R>
+hipError_t launch_adj_fir_rows(const cpx<R> *in, cpx<R> *out, long long nrows, int n,
+                               const R *hz, int K, hipStream_t st) {
+  if (nrows <= 0 || n < 1) return hipSuccess;
+  const size_t lds = 4 * (size_t)(n + 2 * K) * sizeof(cpx<R>);
+  if (lds > 65536) return hipErrorInvalidValue;
+  k_adj_fir_rows<R><<<(unsigned)((nrows + 3) / 4), 256, lds, st>>>(in, out, nrows, n,
+                                                                  hz, K);
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_adj_prefilter_rows(cpx<R> *rows, long long nrows, int n,
                                      int order, hipStream_t st) {
   int np = 0;
@@ -442,6 +570,13 @@ hipError_t launch_adj_fft(int stage, int n, const void *in, void *out,
 }
 
 #define ODT_ADJ_INST(R)                                                         \
+  template hipError_t launch_adj_fir_rows<R>(const cpx<R> *, cpx<R> *,         \
+                                             long long, int, const R *, int,    \
+                                             hipStream_t);                      \
+  template hipError_t launch_adj_fir<R>(const cpx<R> *, cpx<R> *, long long,   \
+                                        int, long long, long long, long long,  \
+                                        long long, const R *, int,             \
+                                        hipStream_t);                          \
   template hipError_t launch_adj_prefilter_rows<R>(cpx<R> *, long long, int,  \
                                                    int, hipStream_t);          \
   template hipError_t launch_adj_rotate<R>(int, const cpx<R> *, int,            \

@@ -90,6 +90,15 @@ hipError_t launch_opt_to_ri(int is_complex, long long n, double res, double nm,
 
 // Adjoint stages (odt_adjoint.hip).
 template <typename R>
+hipError_t launch_adj_fir(const cpx<R> *in, cpx<R> *out, long long nlines, int n,
+                          long long n0, long long s0, long long s1, long long es,
+                          const R *hz, int K, hipStream_t st);
+template <typename R>
+hipError_t launch_adj_fir_rows(const cpx<R> *in, cpx<R> *out, long long nrows, int n,
+                               const R *hz, int K, hipStream_t st);
+// Zero padding on each side of the tap table of launch_adj_fir.
+int adj_fir_pad();
+template <typename R>
 hipError_t launch_adj_prefilter_rows(cpx<R> *rows, long long nrows, int n,
                                      int order, hipStream_t st);
 template <typename R>

@@ -129,6 +129,8 @@ struct odt_plan {
   void *twx = nullptr, *twy = nullptr, *P = nullptr, *aux = nullptr;
   void *q0x = nullptr, *q0y = nullptr, *q0z = nullptr;
   double *prop_a = nullptr, *weights = nullptr, *params = nullptr;
+  void *fir_taps = nullptr;  // adjoint: prefilter impulse response (lazily)
+  int fir_K = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev_rot;  // start/stop pairs
@@ -231,7 +233,7 @@ int per_block(int n, size_t esz, int cap) {
 
 void free_plan(odt_plan *p) {
   void *ptrs[] = {p->twx, p->twy, p->P, p->aux, p->q0x, p->q0y, p->q0z,
-                  p->prop_a, p->weights, p->params};
+                  p->prop_a, p->weights, p->params, p->fir_taps};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto e : p->ev_rot) (void)hipEventDestroy(e);
@@ -327,6 +329,36 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
   return ODT_OK;
 }
 
+// Two-sided impulse response of the mirror-boundary B-spline prefilter of
+// an order: per pole z the factor (1 - z)^2 / ((1 - z q)(1 - z / q)) has
+// the response (1 - z) / (1 + z) z^|k|; the poles' responses are convolved
+// and truncated where they fall below 1e-18 of the centre tap.  h[k + K].
+std::vector<double> bspline_fir(int order, int *K) {
+  int np = 0;
+  double z1 = 0, z2 = 0, gain = 1;
+  odt::iir_poles(order, &np, &z1, &z2, &gain);
+  std::vector<double> h(1, 1.0);
+  for (int pi = 0; pi < np; ++pi) {
+    const double z = pi == 0 ? z1 : z2;
+    const int kp = (int)std::ceil(std::log(1e-18) / std::log(std::fabs(z)));
+    std::vector<double> hp(2 * kp + 1);
+    for (int k = -kp; k <= kp; ++k)
+      hp[k + kp] = (1 - z) / (1 + z) * std::pow(z, (double)std::abs(k));
+    std::vector<double> c(h.size() + hp.size() - 1, 0.0);
+    for (size_t i = 0; i < h.size(); ++i)
+      for (size_t j = 0; j < hp.size(); ++j) c[i + j] += h[i] * hp[j];
+    h.swap(c);
+  }
+  const int kc = (int)(h.size() - 1) / 2;
+  int kk = 0;
+  for (int k = 0; k <= kc; ++k)
+    if (std::fabs(h[kc + k]) > 1e-18 * std::fabs(h[kc]) ||
+        std::fabs(h[kc - k]) > 1e-18 * std::fabs(h[kc]))
+      kk = k;
+  *K = kk;
+  return std::vector<double>(h.begin() + (kc - kk), h.begin() + (kc + kk + 1));
+}
+
 // Adjoint of execute_t for the y-axis reconstruction (see odt_adjoint.hip).
 template <typename R>
 int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
@@ -340,11 +372,37 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
   cpx<R> *gu = (cpx<R> *)d_gusin;
   const long long Nx = g.Nx, Ny = g.Ny, ln = g.ln;
   const size_t per_angle = (size_t)Ny * ln * Nx;
-  for (int b0 = 0; b0 < p->A; b0 += p->NB) {
-    const int nb = std::min(p->NB, p->A - b0);
+  // With room for two half batches the transposed prefilter runs as two
+  // out-of-place convolutions (z into the second half, x back); otherwise
+  // as the in-place recursions.
+  const bool fir = p->NB >= 2;
+  const int step = fir ? p->NB / 2 : p->NB;
+  cpx<R> *slabB = slab + (size_t)step * per_angle;
+  if (fir && !p->fir_taps) {
+    int K = 0;
+    const std::vector<double> h = bspline_fir(p->order, &K);
+    const int pad = odt::adj_fir_pad();
+    std::vector<R> hz(2 * K + 2 * pad, R(0));
+    for (int k = -K; k <= K; ++k) hz[k + K + pad] = (R)h[k + K];
+    HIPCHK(hipMalloc(&p->fir_taps, hz.size() * sizeof(R)), "hipMalloc(taps)");
+    HIPCHK(hipMemcpy(p->fir_taps, hz.data(), hz.size() * sizeof(R),
+                     hipMemcpyHostToDevice),
+           "hipMemcpy(taps)");
+    p->fir_K = K;
+  }
+  for (int b0 = 0; b0 < p->A; b0 += step) {
+    const int nb = std::min(step, p->A - b0);
     HIPCHK(odt::launch_adj_rotate<R>(p->order, gout, nb, p->params + 6 * (size_t)b0,
                                      slab, g.Ny, g.ln, g.Nx, st),
            "k_adj_rotate");
+    if (fir) {
+      const R *hz = (const R *)p->fir_taps;
+      HIPCHK(odt::launch_adj_fir<R>(slab, slabB, nb * Ny * Nx, g.ln, Nx, 1, ln * Nx, Nx,
+                                    hz, p->fir_K, st),
+             "k_adj_fir z");
+      HIPCHK(odt::launch_adj_fir_rows<R>(slabB, slab, nb * Ny * ln, g.Nx, hz, p->fir_K, st),
+             "k_adj_fir_rows");
+    } else {
     // Q^T = D Q D^-1 along x, then along z (real and imaginary lines)
     R *sr = (R *)slab;
     const long long lz = nb * Ny * Nx * 2;
@@ -356,6 +414,7 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
            "iir z");
     HIPCHK(odt::launch_scale_ends<R>(sr, lz, g.ln, Nx * 2, 1, ln * Nx * 2, Nx * 2, 0.5, st),
            "ends z");
+    }
     for (int c0 = b0; c0 < b0 + nb; c0 += p->NC) {
       const int n = std::min(p->NC, b0 + nb - c0);
       HIPCHK(odt::launch_adj_fft<R>(0, g.lNx, slab + (size_t)(c0 - b0) * per_angle, Tp,

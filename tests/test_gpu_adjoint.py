@@ -127,3 +127,27 @@ def test_gradient_descent_decreases_loss_fp32():
         t *= 2.0
     assert all(b < a for a, b in zip(losses, losses[1:])), losses
     assert losses[-1] < 0.5 * losses[0], losses
+
+
+@pytest.mark.parametrize("budget_angles", [1, 3, 5])
+def test_adjoint_slab_budgets_agree(monkeypatch, budget_angles):
+    """A slab budget of one angle runs the transposed prefilter as
+    recursions in place; larger ones as convolutions over half batches
+    (several chunks here).  All give the same adjoint (FP64 to 1e-12) and
+    pass the dot-product test."""
+    A, ny, nx = 9, 20, 24
+    rng = np.random.default_rng(11)
+    angles = np.sort(rng.uniform(0, 2 * np.pi, A))
+    u = _rand((A, ny, nx), np.complex128, 12)
+    g = _rand((nx, ny, nx), np.complex128, 13)
+    full = AG.backpropagate_3d_adjoint(g, angles, 6.0, 1.333, dtype=np.float64)
+    monkeypatch.setenv("ODTBRAIN_AMD_SLAB_BUDGET",
+                       str(budget_angles * nx * ny * nx * 16))
+    part = AG.backpropagate_3d_adjoint(g, angles, 6.0, 1.333,
+                                       dtype=np.float64)
+    err = float(torch.linalg.vector_norm(part - full) /
+                torch.linalg.vector_norm(full))
+    assert err < 1e-12, err
+    Mu = G.backpropagate_3d(u, angles, 6.0, 1.333, dtype=np.float64)
+    lhs, rhs = _dot(g, Mu), _dot(part, u)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
