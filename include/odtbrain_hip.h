@@ -191,6 +191,24 @@ int odt_odt_to_ri(int precision, int is_complex, int64_t n, double res,
 int odt_opt_to_ri(int precision, int is_complex, int64_t n, double res,
                   double nm, const void *d_f, void *d_ri, void *stream);
 
+/* Sinogram preparation (odtbrain/_prepare_sino.py:7-182).
+ * rytov = 1: sinogram_as_rytov, d_out complex (A, [Ny,] Nx) =
+ *   u0 * (log|u/u0| + i * unwrapped angle(u/u0));
+ * rytov = 0: sinogram_as_radon, d_out real = unwrapped angle(u).
+ * ndim 2: (A, Nx) sinogram, numpy.unwrap along the detector axis;
+ * ndim 3: (A, Ny, Nx), every angle's image unwrapped with the reliability-
+ *   sorted algorithm of skimage.restoration.unwrap_phase (Herraez et al.
+ *   2002; border reliabilities fixed, ties in edge creation order).
+ * align = 1: align_unwrapped (_prepare_sino.py:7-46).  d_u0: null for
+ * u0 = 1, else a complex array of the full shape.  Ny ignored for ndim 2.
+ * Workspace from odt_prepare_workspace_size; asynchronous. */
+int odt_prepare_workspace_size(int precision, int ndim, int32_t A, int32_t Ny,
+                               int32_t Nx, int rytov, size_t *bytes);
+int odt_sinogram_prepare(int precision, int ndim, int32_t A, int32_t Ny,
+                         int32_t Nx, const void *d_usin, const void *d_u0,
+                         int rytov, int align, void *d_out, void *d_ws,
+                         size_t ws_bytes, void *stream);
+
 const char *odt_last_error(void);
 int odt_version(void);
 

@@ -22,6 +22,7 @@ EXPORTED = (
     "odt_plan_adjoint",
     "odt_batch_create", "odt_batch_launch", "odt_batch_destroy",
     "odt_odt_to_ri", "odt_opt_to_ri",
+    "odt_prepare_workspace_size", "odt_sinogram_prepare",
     "odt_last_error", "odt_version",
 )
 
@@ -97,6 +98,12 @@ def lib():
         getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_int, i64,
                                      ctypes.c_double, ctypes.c_double, vp, vp,
                                      vp]
+    L.odt_prepare_workspace_size.argtypes = [ctypes.c_int, ctypes.c_int, i32,
+                                             i32, i32, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_size_t)]
+    L.odt_sinogram_prepare.argtypes = [ctypes.c_int, ctypes.c_int, i32, i32,
+                                       i32, vp, vp, ctypes.c_int, ctypes.c_int,
+                                       vp, vp, ctypes.c_size_t, vp]
     L.odt_last_error.restype = ctypes.c_char_p
     L.odt_last_error.argtypes = []
     L.odt_version.argtypes = []

@@ -88,6 +88,16 @@ template <typename R>
 hipError_t launch_opt_to_ri(int is_complex, long long n, double res, double nm,
                             const void *f, void *ri, hipStream_t st);
 
+// Sinogram preparation (odt_prepare.hip): ndim 2 -> (A, W) with H = 1,
+// ndim 3 -> (A, H, W); u0 null for u0 = 1, else broadcast to the full
+// shape; rytov: complex output u0 (log|.| + i phase), else the real phase.
+template <typename R>
+hipError_t prepare_workspace(int ndim, int A, int H, int W, int rytov, size_t *bytes);
+template <typename R>
+hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const void *u0,
+                          int rytov, int align, void *out, void *ws, size_t ws_bytes,
+                          hipStream_t st);
+
 // Adjoint stages (odt_adjoint.hip).
 template <typename R>
 hipError_t launch_adj_fir(const cpx<R> *in, cpx<R> *out, long long nlines, int n,

@@ -787,6 +787,59 @@ int odt_opt_to_ri(int precision, int is_complex, int64_t n, double res, double n
   return ODT_OK;
 }
 
+namespace {
+int prepare_check(int precision, int ndim, int32_t A, int32_t Ny, int32_t Nx) {
+  if (precision != ODT_FP32 && precision != ODT_FP64)
+    return fail(ODT_ERR_INVALID, "precision");
+  if (ndim != 2 && ndim != 3) return fail(ODT_ERR_INVALID, "ndim must be 2 or 3");
+  if (A < 1 || Nx < 3 || (ndim == 3 && Ny < 2))
+    return fail(ODT_ERR_INVALID, "sinogram too small");
+  const long long n = (long long)A * (ndim == 3 ? Ny : 1) * Nx;
+  const long long E = ndim == 3 ? 2LL * Ny * Nx : 0;
+  if (n > (1LL << 31) - 1 || (long long)A * E > 0xffffffffLL)
+    return fail(ODT_ERR_UNSUPPORTED, "sinogram too large");
+  return ODT_OK;
+}
+}  // namespace
+
+int odt_prepare_workspace_size(int precision, int ndim, int32_t A, int32_t Ny,
+                               int32_t Nx, int rytov, size_t *bytes) {
+  if (!bytes) return fail(ODT_ERR_INVALID, "null argument");
+  const int rc = prepare_check(precision, ndim, A, Ny, Nx);
+  if (rc != ODT_OK) return rc;
+  const int H = ndim == 3 ? Ny : 1;
+  hipError_t e = precision == ODT_FP64
+                     ? odt::prepare_workspace<double>(ndim, A, H, Nx, rytov, bytes)
+                     : odt::prepare_workspace<float>(ndim, A, H, Nx, rytov, bytes);
+  if (e != hipSuccess) return hip_fail(e, "prepare_workspace");
+  return ODT_OK;
+}
+
+int odt_sinogram_prepare(int precision, int ndim, int32_t A, int32_t Ny, int32_t Nx,
+                         const void *d_usin, const void *d_u0, int rytov, int align,
+                         void *d_out, void *d_ws, size_t ws_bytes, void *stream) {
+  const int rc = prepare_check(precision, ndim, A, Ny, Nx);
+  if (rc != ODT_OK) return rc;
+  if (!d_usin || !d_out || !d_ws) return fail(ODT_ERR_INVALID, "null argument");
+  size_t need = 0;
+  const int H = ndim == 3 ? Ny : 1;
+  HIPCHK(precision == ODT_FP64 ? odt::prepare_workspace<double>(ndim, A, H, Nx, rytov, &need)
+                               : odt::prepare_workspace<float>(ndim, A, H, Nx, rytov, &need),
+         "prepare_workspace");
+  if (ws_bytes < need)
+    return fail(ODT_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) +
+                                       " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e =
+      precision == ODT_FP64
+          ? odt::launch_prepare<double>(ndim, A, H, Nx, d_usin, d_u0, rytov, align, d_out,
+                                        d_ws, ws_bytes, st)
+          : odt::launch_prepare<float>(ndim, A, H, Nx, d_usin, d_u0, rytov, align, d_out,
+                                       d_ws, ws_bytes, st);
+  if (e != hipSuccess) return hip_fail(e, "odt_sinogram_prepare");
+  return ODT_OK;
+}
+
 int odt_spline_prefilter(int precision, int order, int64_t nlines, int32_t n,
                          int64_t line_stride, int64_t elem_stride, void *d_data,
                          void *stream) {

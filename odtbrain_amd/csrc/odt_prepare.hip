@@ -1,0 +1,510 @@
+// Sinogram preparation (odtbrain/_prepare_sino.py:7-182): the Rytov (or
+// plain phase) sinogram of a complex field, with phase unwrapping and
+// alignment.
+//
+//   k_prep_phase       angle(u / u0) and log|u / u0| per sample (numpy's
+//                      Smith complex division when u0 is an array)
+//   k_unwrap_rows      2D sinograms: numpy.unwrap along the detector axis,
+//                      one thread per row, the reference's sequential sum
+//   3D sinograms: every angle's image unwrapped by the reliability-sorted
+//   algorithm of Herraez et al. (Applied Optics 41 (2002) 7437), the
+//   algorithm of skimage.restoration.unwrap_phase (_prepare_sino.py:172):
+//   k_uw_reliability   wrapped second differences per pixel (borders: 1e7)
+//   k_uw_edges         4-neighbour edges, key = sum of the two reliabilities
+//   segmented radix sort (rocPRIM, stable: equal keys keep creation order)
+//   k_uw_gather        one thread per image merges groups edge by edge in
+//                      sorted order (union by size with whole-turn offsets,
+//                      the published merge rules)
+//   k_uw_apply         value + 2 pi * increment
+//   k_min_*, k_align   align_unwrapped: steps of the unwrapped corner
+//                      samples along the angles, their mode per angle,
+//                      whole turns from the minimum (divmod_neg)
+//   k_finish           u0 * (log|.| + i (phase - remove)) or phase - remove
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include "odt_device.h"
+#include "odt_launch.h"
+
+namespace odt {
+
+namespace {
+constexpr double kPi = 3.141592653589793;
+constexpr double kTwoPi = 6.283185307179586;
+constexpr double kBorderReliability = 1e7;
+}  // namespace
+
+// numpy's complex division (Smith's method, as in its loops), no FMA
+template <typename R>
+__device__ __forceinline__ cpx<R> np_cdiv(cpx<R> a, cpx<R> b) {
+#pragma clang fp contract(off)
+  const R ar = fabs(b.re), ai = fabs(b.im);
+  if (ar >= ai) {
+    if (ar == R(0) && ai == R(0)) return mk<R>(a.re / ar, a.im / ai);
+    const R rat = b.im / b.re, scl = R(1) / (b.re + b.im * rat);
+    return mk<R>((a.re + a.im * rat) * scl, (a.im - a.re * rat) * scl);
+  }
+  const R rat = b.re / b.im, scl = R(1) / (b.im + b.re * rat);
+  return mk<R>((a.re * rat + a.im) * scl, (a.im * rat - a.re) * scl);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_prep_phase(const cpx<R> *__restrict__ u, const cpx<R> *__restrict__ u0,
+                 long long n, R *__restrict__ phase, R *__restrict__ lna) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cpx<R> q = u[i];
+  if (u0) q = np_cdiv(q, u0[i]);
+  phase[i] = atan2(q.im, q.re);
+  if (lna) lna[i] = log(hypot(q.re, q.im));
+}
+
+// numpy.mod for floats (result takes the divisor's sign)
+template <typename R>
+__device__ __forceinline__ R np_mod(R a, R b) {
+  R m = fmod(a, b);
+  if (m != R(0)) {
+    if ((b < R(0)) != (m < R(0))) m += b;
+  } else {
+    m = copysign(R(0), b);
+  }
+  return m;
+}
+
+// numpy.unwrap (period 2 pi, discont pi) of n values at stride s, in place:
+// corrections from the differences of the original values, summed in order.
+template <typename R>
+__device__ void np_unwrap_line(R *p, int n, long long s) {
+#pragma clang fp contract(off)
+  const R pi = (R)kPi, period = (R)kTwoPi;
+  R prev = p[0], cum = R(0);
+  for (int i = 1; i < n; ++i) {
+    const R cur = p[(long long)i * s];
+    const R dd = cur - prev;
+    R ddmod = np_mod<R>(dd - (-pi), period) + (-pi);
+    if (ddmod == -pi && dd > R(0)) ddmod = pi;
+    R pc = ddmod - dd;
+    if (fabs(dd) < pi) pc = R(0);
+    cum = cum + pc;
+    p[(long long)i * s] = cur + cum;
+    prev = cur;
+  }
+}
+
+template <typename R>
+__global__ void k_unwrap_rows(R *phase, int rows, int n) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) np_unwrap_line<R>(phase + (long long)r * n, n, 1);
+}
+
+__device__ __forceinline__ double uw_wrap(double d) {
+  if (d > kPi) return d - kTwoPi;
+  if (d < -kPi) return d + kTwoPi;
+  return d;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_uw_reliability(const R *__restrict__ phase, int A, int H, int W,
+                     double *__restrict__ rel) {
+#pragma clang fp contract(off)
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long npix = (long long)H * W;
+  if (i >= (long long)A * npix) return;
+  const long long a = i / npix;
+  const int p = (int)(i - a * npix), y = p / W, x = p - y * W;
+  if (y == 0 || y == H - 1 || x == 0 || x == W - 1) {
+    rel[i] = kBorderReliability;
+    return;
+  }
+  const R *im = phase + a * npix;
+  auto v = [&](int yy, int xx) { return (double)im[yy * W + xx]; };
+  const double c = v(y, x);
+  const double h = uw_wrap(v(y, x - 1) - c) - uw_wrap(c - v(y, x + 1));
+  const double vv = uw_wrap(v(y - 1, x) - c) - uw_wrap(c - v(y + 1, x));
+  const double d1 = uw_wrap(v(y - 1, x - 1) - c) - uw_wrap(c - v(y + 1, x + 1));
+  const double d2 = uw_wrap(v(y - 1, x + 1) - c) - uw_wrap(c - v(y + 1, x - 1));
+  rel[i] = h * h + vv * vv + d1 * d1 + d2 * d2;
+}
+
+// Edge e of an image: e < H (W - 1) horizontal (y, x)-(y, x + 1) row by
+// row, then vertical (y, x)-(y + 1, x).
+__device__ __forceinline__ void uw_edge(int e, int H, int W, int *p1, int *p2) {
+  const int nh = H * (W - 1);
+  if (e < nh) {
+    const int y = e / (W - 1), x = e - y * (W - 1);
+    *p1 = y * W + x;
+    *p2 = *p1 + 1;
+  } else {
+    *p1 = e - nh;
+    *p2 = *p1 + W;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_uw_edges(const double *__restrict__ rel, int A, int H, int W,
+               double *__restrict__ keys, int *__restrict__ vals,
+               unsigned *__restrict__ offsets) {
+  const long long E = (long long)H * (W - 1) + (long long)(H - 1) * W;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= A && i >= 0) {
+    if (i <= A) offsets[i] = (unsigned)(i * E);
+  }
+  if (i >= (long long)A * E) return;
+  const long long a = i / E;
+  const int e = (int)(i - a * E);
+  int p1, p2;
+  uw_edge(e, H, W, &p1, &p2);
+  const double *r = rel + a * (long long)H * W;
+  keys[i] = r[p1] + r[p2];
+  vals[i] = e;
+}
+
+__device__ __forceinline__ int uw_find_wrap(double v1, double v2) {
+  const double d = v1 - v2;
+  if (d > kPi) return -1;
+  if (d < -kPi) return 1;
+  return 0;
+}
+
+// Root of p and p's increment (offsets to the root plus the root's own
+// increment), compressing the path.
+__device__ __forceinline__ int uw_find(int p, int *parent, int *off, int *inc) {
+  int r = p, s = 0;
+  while (parent[r] != r) {
+    s += off[r];
+    r = parent[r];
+  }
+  // compress: every node on the path points at r with its total offset
+  int q = p, sq = s;
+  while (parent[q] != r && q != r) {
+    const int nq = parent[q], oq = off[q];
+    parent[q] = r;
+    off[q] = sq;
+    sq -= oq;
+    q = nq;
+  }
+  *inc = s + off[r];
+  return r;
+}
+
+// One thread per image: the groups of the published algorithm as a
+// union-find forest.  parent/off/size per pixel; a root's off is its own
+// increment, any other node's off is relative to its parent.
+template <typename R>
+__global__ void k_uw_gather(const R *__restrict__ phase, int A, int H, int W,
+                            const int *__restrict__ sorted, int *parent_all,
+                            int *off_all, int *size_all) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A) return;
+  const long long npix = (long long)H * W;
+  const long long E = (long long)H * (W - 1) + (long long)(H - 1) * W;
+  const R *img = phase + a * npix;
+  int *parent = parent_all + a * npix, *off = off_all + a * npix,
+      *size = size_all + a * npix;
+  for (int p = 0; p < npix; ++p) {
+    parent[p] = p;
+    off[p] = 0;
+    size[p] = 1;
+  }
+  const int *es = sorted + a * E;
+  for (long long k = 0; k < E; ++k) {
+    int p1, p2;
+    uw_edge(es[k], H, W, &p1, &p2);
+    int i1, i2;
+    const int r1 = uw_find(p1, parent, off, &i1);
+    const int r2 = uw_find(p2, parent, off, &i2);
+    if (r1 == r2) continue;
+    const int e = uw_find_wrap((double)img[p1], (double)img[p2]);
+    if (size[r2] == 1) {  // p2 alone joins p1's group
+      parent[p2] = r1;
+      off[p2] = i1 - e - off[r1];
+      size[r1] += 1;
+    } else if (size[r1] == 1) {  // p1 alone joins p2's group
+      parent[p1] = r2;
+      off[p1] = i2 + e - off[r2];
+      size[r2] += 1;
+    } else if (size[r1] > size[r2]) {  // group 2 shifted onto group 1
+      const int d = i1 - e - i2;
+      off[r2] = off[r2] + d - off[r1];
+      parent[r2] = r1;
+      size[r1] += size[r2];
+    } else {  // group 1 shifted onto group 2
+      const int d = i2 + e - i1;
+      off[r1] = off[r1] + d - off[r2];
+      parent[r1] = r2;
+      size[r2] += size[r1];
+    }
+  }
+}
+
+// value + 2 pi * increment, in float64 as the published code, stored as R
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_uw_apply(R *phase, int A, int H, int W, const int *__restrict__ parent_all,
+               const int *__restrict__ off_all) {
+#pragma clang fp contract(off)
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long npix = (long long)H * W;
+  if (i >= (long long)A * npix) return;
+  const long long a = i / npix;
+  const int *parent = parent_all + a * npix, *off = off_all + a * npix;
+  int r = (int)(i - a * npix), s = 0;
+  while (parent[r] != r) {
+    s += off[r];
+    r = parent[r];
+  }
+  s += off[r];
+  phase[i] = (R)((double)phase[i] + kTwoPi * (double)s);
+}
+
+// minimum of n values: per-block minima, then one block over them
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_min_blocks(const R *__restrict__ v, long long n, R *__restrict__ part) {
+  __shared__ R sm[256];
+  R m = R(INFINITY);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const R x = v[i];
+    m = (x < m || isnan(x)) ? x : m;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const R o = sm[threadIdx.x + s];
+      if (o < sm[threadIdx.x] || isnan(o)) sm[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// numpy / Python float divmod quotient, then divmod_neg's move of the
+// remainder towards zero (_prepare_sino.py:49-57)
+__device__ double divmod_neg_q(double a, double b) {
+#pragma clang fp contract(off)
+  double mod = fmod(a, b);
+  double div = (a - mod) / b;
+  if (mod != 0.0) {
+    if ((b < 0.0) != (mod < 0.0)) {
+      mod += b;
+      div -= 1.0;
+    }
+  } else {
+    mod = copysign(0.0, b);
+  }
+  double q;
+  if (div != 0.0) {
+    q = floor(div);
+    if (div - q > 0.5) q += 1.0;
+  } else {
+    q = copysign(0.0, a / b);
+  }
+  const double sr = mod > 0.0 ? 1.0 : (mod < 0.0 ? -1.0 : 0.0);
+  if (fabs(mod) > b / 2) q += sr;
+  return q;
+}
+
+// align_unwrapped: remove[a] = mode over the 5 corner samples of
+// (sample - unwrap(sample)) at angle a, plus whole turns from the minimum.
+// One block; steps (5 x A) in scratch.
+template <typename R>
+__global__ void k_align(const R *__restrict__ phase, int A, int ndim, int H,
+                        int W, const R *__restrict__ part, int nparts,
+                        R *__restrict__ samp, double *__restrict__ steps,
+                        double *__restrict__ remove) {
+#pragma clang fp contract(off)
+  const long long npix = (long long)H * W;
+  const int t = threadIdx.x;
+  if (t < 5) {
+    // 3D: (0,0) (0,-1) (-1,0) (-1,-1) (0,1); 2D: columns 0 1 2 -1 -2
+    long long o;
+    if (ndim == 3) {
+      const int ys[5] = {0, 0, H - 1, H - 1, 0};
+      const int xs[5] = {0, W - 1, 0, W - 1, 1};
+      o = (long long)ys[t] * W + xs[t];
+    } else {
+      const int cs[5] = {0, 1, 2, W - 1, W - 2};
+      o = cs[t];
+    }
+    R *s = samp + (long long)t * A;
+    for (int a = 0; a < A; ++a) s[a] = phase[a * npix + o];
+    np_unwrap_line<R>(s, A, 1);
+    for (int a = 0; a < A; ++a)
+      steps[(long long)t * A + a] = (double)(R)(phase[a * npix + o] - s[a]);
+  }
+  __syncthreads();
+  __shared__ double qmin;
+  if (t == 0) {
+    R m = R(INFINITY);
+    for (int i = 0; i < nparts; ++i) m = (part[i] < m || isnan(part[i])) ? part[i] : m;
+    qmin = divmod_neg_q((double)m, kTwoPi);
+  }
+  __syncthreads();
+  for (int a = t; a < A; a += blockDim.x) {
+    // scipy.stats.mode: the most frequent value, the smallest on ties
+    double best = 0.0;
+    int bestc = 0;
+    for (int i = 0; i < 5; ++i) {
+      const double v = steps[(long long)i * A + a];
+      int c = 0;
+      for (int j = 0; j < 5; ++j) c += steps[(long long)j * A + a] == v;
+      if (c > bestc || (c == bestc && v < best)) {
+        best = v;
+        bestc = c;
+      }
+    }
+    remove[a] = best + qmin * kTwoPi;
+  }
+}
+
+// out = u0 * (lna + i (phase - remove)) (Rytov) or phase - remove (radon)
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_finish(const R *__restrict__ phase, const R *__restrict__ lna,
+             const cpx<R> *__restrict__ u0, const double *__restrict__ remove,
+             long long n, long long per_angle, void *out) {
+#pragma clang fp contract(off)
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R ph = phase[i];
+  if (remove) ph = (R)((double)ph - remove[i / per_angle]);
+  if (!lna) {
+    reinterpret_cast<R *>(out)[i] = ph;
+    return;
+  }
+  cpx<R> r = mk<R>(lna[i], ph);
+  if (u0) {
+    const cpx<R> w = u0[i];
+    r = mk<R>(w.re * r.re - w.im * r.im, w.re * r.im + w.im * r.re);
+  }
+  reinterpret_cast<cpx<R> *>(out)[i] = r;
+}
+
+namespace {
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+unsigned blocks_of(long long n) { return (unsigned)((n + 255) / 256); }
+constexpr int kMinParts = 1024;
+
+struct PrepLayout {
+  size_t phase, lna, rel, keys0, keys1, vals0, vals1, offs, parent, off, size,
+      part, samp, steps, remove, sort_tmp, total;
+  size_t sort_bytes;
+};
+
+template <typename R>
+hipError_t prep_layout(int ndim, int A, int H, int W, bool rytov, PrepLayout *L) {
+  const long long n = (long long)A * H * W;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = al256(o + bytes);
+    return at;
+  };
+  L->phase = take(n * sizeof(R));
+  L->lna = rytov ? take(n * sizeof(R)) : 0;
+  const long long E = ndim == 3 ? (long long)H * (W - 1) + (long long)(H - 1) * W : 0;
+  L->sort_bytes = 0;
+  if (ndim == 3) {
+    L->rel = take(n * sizeof(double));
+    L->keys0 = take(A * E * sizeof(double));
+    L->keys1 = take(A * E * sizeof(double));
+    L->vals0 = take(A * E * sizeof(int));
+    L->vals1 = take(A * E * sizeof(int));
+    L->offs = take((A + 1) * sizeof(unsigned));
+    L->parent = take(n * sizeof(int));
+    L->off = take(n * sizeof(int));
+    L->size = take(n * sizeof(int));
+    hipError_t e = rocprim::segmented_radix_sort_pairs(
+        nullptr, L->sort_bytes, (double *)nullptr, (double *)nullptr, (int *)nullptr,
+        (int *)nullptr, (unsigned)(A * E), (unsigned)A, (unsigned *)nullptr,
+        (unsigned *)nullptr + 1);
+    if (e != hipSuccess) return e;
+    L->sort_tmp = take(L->sort_bytes);
+  }
+  L->part = take(kMinParts * sizeof(R));
+  L->samp = take(5 * (size_t)A * sizeof(R));
+  L->steps = take(5 * (size_t)A * sizeof(double));
+  L->remove = take((size_t)A * sizeof(double));
+  L->total = o;
+  return hipSuccess;
+}
+}  // namespace
+
+template <typename R>
+hipError_t prepare_workspace(int ndim, int A, int H, int W, int rytov, size_t *bytes) {
+  PrepLayout L;
+  hipError_t e = prep_layout<R>(ndim, A, H, W, rytov != 0, &L);
+  *bytes = L.total;
+  return e;
+}
+
+template <typename R>
+hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const void *u0,
+                          int rytov, int align, void *out, void *ws, size_t ws_bytes,
+                          hipStream_t st) {
+  PrepLayout L;
+  hipError_t e = prep_layout<R>(ndim, A, H, W, rytov != 0, &L);
+  if (e != hipSuccess) return e;
+  if (ws_bytes < L.total) return hipErrorInvalidValue;
+  char *w = (char *)ws;
+  const long long n = (long long)A * H * W;
+  R *phase = (R *)(w + L.phase);
+  R *lna = rytov ? (R *)(w + L.lna) : nullptr;
+  k_prep_phase<R><<<blocks_of(n), 256, 0, st>>>((const cpx<R> *)u, (const cpx<R> *)u0, n,
+                                                 phase, lna);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (ndim == 2) {
+    k_unwrap_rows<R><<<(A + 63) / 64, 64, 0, st>>>(phase, A, W);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  } else {
+    const long long E = (long long)H * (W - 1) + (long long)(H - 1) * W;
+    double *rel = (double *)(w + L.rel);
+    k_uw_reliability<R><<<blocks_of(n), 256, 0, st>>>(phase, A, H, W, rel);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    double *k0 = (double *)(w + L.keys0), *k1 = (double *)(w + L.keys1);
+    int *v0 = (int *)(w + L.vals0), *v1 = (int *)(w + L.vals1);
+    unsigned *offs = (unsigned *)(w + L.offs);
+    const long long ne = std::max<long long>((long long)A * E, (long long)A + 1);
+    k_uw_edges<<<blocks_of(ne), 256, 0, st>>>(rel, A, H, W, k0, v0, offs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    size_t sb = L.sort_bytes;
+    e = rocprim::segmented_radix_sort_pairs(w + L.sort_tmp, sb, k0, k1, v0, v1,
+                                            (unsigned)(A * E), (unsigned)A, offs, offs + 1,
+                                            0, 64, st);
+    if (e != hipSuccess) return e;
+    int *parent = (int *)(w + L.parent), *off = (int *)(w + L.off),
+        *size = (int *)(w + L.size);
+    k_uw_gather<R><<<(A + 63) / 64, 64, 0, st>>>(phase, A, H, W, v1, parent, off, size);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_uw_apply<R><<<blocks_of(n), 256, 0, st>>>(phase, A, H, W, parent, off);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  double *remove = nullptr;
+  if (align) {
+    R *part = (R *)(w + L.part);
+    const int nparts = (int)std::min<long long>(kMinParts, (n + 255) / 256);
+    k_min_blocks<R><<<nparts, 256, 0, st>>>(phase, n, part);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    remove = (double *)(w + L.remove);
+    k_align<R><<<1, 256, 0, st>>>(phase, A, ndim, H, W, part, nparts, (R *)(w + L.samp),
+                                  (double *)(w + L.steps), remove);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  k_finish<R><<<blocks_of(n), 256, 0, st>>>(phase, lna, (const cpx<R> *)u0, remove, n,
+                                            (long long)H * W, out);
+  return hipGetLastError();
+}
+
+template hipError_t prepare_workspace<float>(int, int, int, int, int, size_t *);
+template hipError_t prepare_workspace<double>(int, int, int, int, int, size_t *);
+template hipError_t launch_prepare<float>(int, int, int, int, const void *, const void *,
+                                          int, int, void *, void *, size_t, hipStream_t);
+template hipError_t launch_prepare<double>(int, int, int, int, const void *, const void *,
+                                           int, int, void *, void *, size_t, hipStream_t);
+
+}  // namespace odt

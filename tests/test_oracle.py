@@ -211,3 +211,94 @@ def test_odt_to_ri_oracle_inverts_the_object_function():
     assert np.allclose(T.opt_to_ri(g, res, nm),
                        nm + g * res / (2 * np.pi))
     assert T.opt_to_ri(np.zeros(1), res, nm)[0] == nm
+
+
+# --------------------------------------------------------------------------
+# sinogram_as_rytov / sinogram_as_radon (odtbrain/_prepare_sino.py)
+# --------------------------------------------------------------------------
+def _mod_rest(a, b):
+    """Remainder closest to zero (tests/test_processing.py:35-48)."""
+    q = np.array(a / b, dtype=int)
+    r = a - b * q
+    wrong = np.where(np.abs(r) > b / 2)
+    r[wrong] -= b * np.sign(r[wrong])
+    return r
+
+
+def test_prepare_oracle_fixtures():
+    """tests/test_processing.py:79-137: the fixtures (with the reference
+    test's own 2 pi allowance) and the 2D-vs-3D consistency."""
+    from oracle import prepare as P
+    twopi = 2 * np.pi
+    sino = P.processing_test_sino()
+    rad = P.sinogram_as_radon(sino)
+    if rad.min() > twopi:
+        rad -= twopi
+    assert np.allclose(np.array(rad).flatten().view(float),
+                       load_golden("processing__test_sino_radon"))
+    for sl in ((slice(None), slice(None), 0), (slice(None), 0, slice(None))):
+        r2 = P.sinogram_as_radon(sino[sl])
+        assert np.allclose(0, _mod_rest(r2 - rad[sl], twopi), atol=1e-6)
+    sino = P.processing_test_sino(rytov=True)
+    ryt = P.sinogram_as_rytov(sino)
+    if ryt.imag.min() > twopi:
+        ryt.imag -= twopi
+    assert np.allclose(np.array(ryt).flatten().view(float),
+                       load_golden("processing__test_sino_rytov"))
+    for sl in ((slice(None), slice(None), 0), (slice(None), 0, slice(None))):
+        r2 = P.sinogram_as_rytov(sino[sl])
+        d = r2 - ryt[sl]
+        assert np.allclose(0, (d.real + 1j * _mod_rest(d.imag, twopi))
+                           .view(float), atol=1e-6)
+
+
+def test_unwrap_2d_residue_free_fields_are_recovered():
+    """Smooth phase fields spanning many turns: wrapping then unwrapping
+    gives the field back up to one whole-turn offset (a wrong increment sign,
+    a dropped merge shift or a swapped neighbour fails this)."""
+    from oracle import prepare as P
+    rng = np.random.default_rng(5)
+    for H, W in ((9, 11), (16, 13)):
+        y, x = np.mgrid[0:H, 0:W]
+        phi = np.zeros((H, W))
+        for _ in range(3):
+            cy, cx = rng.uniform(0, H), rng.uniform(0, W)
+            s = rng.uniform(2, 6)
+            phi += rng.uniform(-12, 12) * np.exp(-((y - cy) ** 2 +
+                                                  (x - cx) ** 2) / (2 * s * s))
+        phi += 0.7 * x - 0.4 * y
+        # residue free: neighbouring differences scaled to at most 2.5 rad
+        md = max(np.abs(np.diff(phi, axis=0)).max(),
+                 np.abs(np.diff(phi, axis=1)).max())
+        phi *= 2.5 / md
+        assert np.ptp(phi) > 3 * np.pi
+        wrapped = np.angle(np.exp(1j * phi))
+        un = P.unwrap_2d(wrapped)
+        k = (un - phi) / (2 * np.pi)
+        assert np.allclose(k, np.round(k[0, 0]), atol=1e-9)
+
+
+def test_unwrap_2d_merge_rules_known_answer():
+    """A 1x4 strip has three horizontal edges of equal reliability (all
+    border pixels): they merge in creation order, each singleton joining
+    the group of its left neighbour, so the first pixel keeps its value."""
+    from oracle import prepare as P
+    wrapped = np.array([[3.0, -3.0, 3.1, -3.1]])
+    un = P.unwrap_2d(wrapped)
+    assert np.allclose(un, [[3.0, -3.0 + 2 * np.pi, 3.1, -3.1 + 2 * np.pi]])
+
+
+def test_align_and_divmod_neg_known_answers():
+    from oracle import prepare as P
+    twopi = 2 * np.pi
+    assert P.divmod_neg(5.0, twopi) == (1.0, 5.0 - twopi)
+    assert P.divmod_neg(-4.0, twopi) == (-1.0, -4.0 + twopi)
+    assert P.divmod_neg(3.0, twopi) == (0.0, 3.0)
+    # a ramp over the angles with whole-turn jumps at every corner sample:
+    # alignment removes the jumps and the turns of the minimum
+    A, N = 6, 8
+    base = np.linspace(0.1, 0.6, A)[:, None] + np.zeros((A, N))
+    jumps = twopi * np.array([0, 1, 1, 2, 2, 2])[:, None]
+    sino = base + jumps + 3 * twopi
+    P.align_unwrapped(sino)
+    assert np.allclose(sino, base)
