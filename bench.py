@@ -326,6 +326,21 @@ def main():
     log(f"bench: fp32 {t_step * 1e3:.1f} ms/step, rotate {rot_stage_ms:.2f} ms "
         f"in {launches} launches, plan total {np.mean(tot):.1f} ms")
 
+    # Prologue before the path: sinogram_as_rytov of the complex field
+    # exp(uB) of the same workload (odtbrain/_prepare_sino.py:105-182:
+    # phase and log, per-angle 2D unwrapping, alignment), complex64.
+    field = torch.exp(d_in)
+    odtbrain_amd.sinogram_as_rytov(field)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        ryt = odtbrain_amd.sinogram_as_rytov(field)
+    torch.cuda.synchronize()
+    prologue = {"function": "sinogram_as_rytov",
+                "ms": (time.perf_counter() - t0) / 2 * 1e3,
+                "phase_max_abs_vs_uB": float((ryt.imag - d_in.imag).abs().max())}
+    del field, ryt
+
     # Epilogue after the path: odt_to_ri of the 256^3 complex64 volume
     # (odtbrain/_translate_ri.py:5-57), one elementwise HBM stream, timed
     # with events on the stream it runs on; 16 B per voxel (read + write).
@@ -399,6 +414,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,
+        "prologue": prologue,
         "epilogue": epilogue,
     }
     print(json.dumps(line), flush=True)

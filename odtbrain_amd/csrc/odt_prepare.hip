@@ -129,6 +129,13 @@ __global__ void __launch_bounds__(256)
   rel[i] = h * h + vv * vv + d1 * d1 + d2 * d2;
 }
 
+__device__ __forceinline__ int uw_find_wrap(double v1, double v2) {
+  const double d = v1 - v2;
+  if (d > kPi) return -1;
+  if (d < -kPi) return 1;
+  return 0;
+}
+
 // Edge e of an image: e < H (W - 1) horizontal (y, x)-(y, x + 1) row by
 // row, then vertical (y, x)-(y + 1, x).
 __device__ __forceinline__ void uw_edge(int e, int H, int W, int *p1, int *p2) {
@@ -143,9 +150,10 @@ __device__ __forceinline__ void uw_edge(int e, int H, int W, int *p1, int *p2) {
   }
 }
 
+template <typename R>
 __global__ void __launch_bounds__(256)
-    k_uw_edges(const double *__restrict__ rel, int A, int H, int W,
-               double *__restrict__ keys, int *__restrict__ vals,
+    k_uw_edges(const double *__restrict__ rel, const R *__restrict__ phase, int A,
+               int H, int W, double *__restrict__ keys, int *__restrict__ vals,
                unsigned *__restrict__ offsets) {
   const long long E = (long long)H * (W - 1) + (long long)(H - 1) * W;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,83 +167,79 @@ __global__ void __launch_bounds__(256)
   uw_edge(e, H, W, &p1, &p2);
   const double *r = rel + a * (long long)H * W;
   keys[i] = r[p1] + r[p2];
-  vals[i] = e;
+  // edge id and its whole-turn step (find_wrap + 1 in the low two bits)
+  const R *img = phase + a * (long long)H * W;
+  vals[i] = (e << 2) | (uw_find_wrap((double)img[p1], (double)img[p2]) + 1);
 }
 
-__device__ __forceinline__ int uw_find_wrap(double v1, double v2) {
-  const double d = v1 - v2;
-  if (d > kPi) return -1;
-  if (d < -kPi) return 1;
-  return 0;
-}
-
-// Root of p and p's increment (offsets to the root plus the root's own
-// increment), compressing the path.
-__device__ __forceinline__ int uw_find(int p, int *parent, int *off, int *inc) {
+// Union-find node per pixel: a root holds (-size of its group, its own
+// increment), any other node (parent, its increment relative to the
+// parent).  find: root of p and p's increment, compressing the path.
+__device__ __forceinline__ int uw_find(int p, int2 *node, int *inc, int2 *root) {
   int r = p, s = 0;
-  while (parent[r] != r) {
-    s += off[r];
-    r = parent[r];
+  int2 nr = node[r];
+  while (nr.x >= 0) {
+    s += nr.y;
+    r = nr.x;
+    nr = node[r];
   }
   // compress: every node on the path points at r with its total offset
   int q = p, sq = s;
-  while (parent[q] != r && q != r) {
-    const int nq = parent[q], oq = off[q];
-    parent[q] = r;
-    off[q] = sq;
-    sq -= oq;
-    q = nq;
+  while (q != r) {
+    const int2 nq = node[q];
+    if (nq.x == r) break;
+    node[q] = make_int2(r, sq);
+    sq -= nq.y;
+    q = nq.x;
   }
-  *inc = s + off[r];
+  *inc = s + nr.y;
+  *root = nr;
   return r;
 }
 
 // One thread per image: the groups of the published algorithm as a
-// union-find forest.  parent/off/size per pixel; a root's off is its own
-// increment, any other node's off is relative to its parent.
+// union-find forest (union by size).  The sorted edges carry their
+// whole-turn step and are read 16 at a time ahead of use.
 template <typename R>
-__global__ void k_uw_gather(const R *__restrict__ phase, int A, int H, int W,
-                            const int *__restrict__ sorted, int *parent_all,
-                            int *off_all, int *size_all) {
+__global__ void k_uw_gather(int A, int H, int W, const int *__restrict__ sorted,
+                            int2 *node_all) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= A) return;
   const long long npix = (long long)H * W;
   const long long E = (long long)H * (W - 1) + (long long)(H - 1) * W;
-  const R *img = phase + a * npix;
-  int *parent = parent_all + a * npix, *off = off_all + a * npix,
-      *size = size_all + a * npix;
-  for (int p = 0; p < npix; ++p) {
-    parent[p] = p;
-    off[p] = 0;
-    size[p] = 1;
-  }
+  int2 *node = node_all + a * npix;
+  for (int p = 0; p < npix; ++p) node[p] = make_int2(-1, 0);
   const int *es = sorted + a * E;
-  for (long long k = 0; k < E; ++k) {
-    int p1, p2;
-    uw_edge(es[k], H, W, &p1, &p2);
-    int i1, i2;
-    const int r1 = uw_find(p1, parent, off, &i1);
-    const int r2 = uw_find(p2, parent, off, &i2);
-    if (r1 == r2) continue;
-    const int e = uw_find_wrap((double)img[p1], (double)img[p2]);
-    if (size[r2] == 1) {  // p2 alone joins p1's group
-      parent[p2] = r1;
-      off[p2] = i1 - e - off[r1];
-      size[r1] += 1;
-    } else if (size[r1] == 1) {  // p1 alone joins p2's group
-      parent[p1] = r2;
-      off[p1] = i2 + e - off[r2];
-      size[r2] += 1;
-    } else if (size[r1] > size[r2]) {  // group 2 shifted onto group 1
-      const int d = i1 - e - i2;
-      off[r2] = off[r2] + d - off[r1];
-      parent[r2] = r1;
-      size[r1] += size[r2];
-    } else {  // group 1 shifted onto group 2
-      const int d = i2 + e - i1;
-      off[r1] = off[r1] + d - off[r2];
-      parent[r1] = r2;
-      size[r2] += size[r1];
+  constexpr int CH = 16;
+  for (long long k0 = 0; k0 < E; k0 += CH) {
+    int ev[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) ev[u] = k0 + u < E ? es[k0 + u] : -1;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (ev[u] < 0) break;
+      int p1, p2;
+      uw_edge(ev[u] >> 2, H, W, &p1, &p2);
+      const int e = (ev[u] & 3) - 1;
+      int i1, i2;
+      int2 n1, n2;
+      const int r1 = uw_find(p1, node, &i1, &n1);
+      const int r2 = uw_find(p2, node, &i2, &n2);
+      if (r1 == r2) continue;
+      const int s1 = -n1.x, s2 = -n2.x;
+      if (s2 == 1) {  // p2 alone joins p1's group
+        node[p2] = make_int2(r1, i1 - e - n1.y);
+        node[r1] = make_int2(-(s1 + 1), n1.y);
+      } else if (s1 == 1) {  // p1 alone joins p2's group
+        node[p1] = make_int2(r2, i2 + e - n2.y);
+        node[r2] = make_int2(-(s2 + 1), n2.y);
+      } else if (s1 > s2) {  // group 2 shifted onto group 1
+        node[r2] = make_int2(r1, n2.y + (i1 - e - i2) - n1.y);
+        node[r1] = make_int2(-(s1 + s2), n1.y);
+      } else {  // group 1 shifted onto group 2
+        node[r1] = make_int2(r2, n1.y + (i2 + e - i1) - n2.y);
+        node[r2] = make_int2(-(s1 + s2), n2.y);
+      }
     }
   }
 }
@@ -243,20 +247,21 @@ __global__ void k_uw_gather(const R *__restrict__ phase, int A, int H, int W,
 // value + 2 pi * increment, in float64 as the published code, stored as R
 template <typename R>
 __global__ void __launch_bounds__(256)
-    k_uw_apply(R *phase, int A, int H, int W, const int *__restrict__ parent_all,
-               const int *__restrict__ off_all) {
+    k_uw_apply(R *phase, int A, int H, int W, const int2 *__restrict__ node_all) {
 #pragma clang fp contract(off)
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long npix = (long long)H * W;
   if (i >= (long long)A * npix) return;
   const long long a = i / npix;
-  const int *parent = parent_all + a * npix, *off = off_all + a * npix;
+  const int2 *node = node_all + a * npix;
   int r = (int)(i - a * npix), s = 0;
-  while (parent[r] != r) {
-    s += off[r];
-    r = parent[r];
+  int2 nr = node[r];
+  while (nr.x >= 0) {
+    s += nr.y;
+    r = nr.x;
+    nr = node[r];
   }
-  s += off[r];
+  s += nr.y;
   phase[i] = (R)((double)phase[i] + kTwoPi * (double)s);
 }
 
@@ -416,9 +421,9 @@ hipError_t prep_layout(int ndim, int A, int H, int W, bool rytov, PrepLayout *L)
     L->vals0 = take(A * E * sizeof(int));
     L->vals1 = take(A * E * sizeof(int));
     L->offs = take((A + 1) * sizeof(unsigned));
-    L->parent = take(n * sizeof(int));
-    L->off = take(n * sizeof(int));
-    L->size = take(n * sizeof(int));
+    L->parent = take(n * sizeof(int2));
+    L->off = 0;
+    L->size = 0;
     hipError_t e = rocprim::segmented_radix_sort_pairs(
         nullptr, L->sort_bytes, (double *)nullptr, (double *)nullptr, (int *)nullptr,
         (int *)nullptr, (unsigned)(A * E), (unsigned)A, (unsigned *)nullptr,
@@ -470,18 +475,19 @@ hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const vo
     int *v0 = (int *)(w + L.vals0), *v1 = (int *)(w + L.vals1);
     unsigned *offs = (unsigned *)(w + L.offs);
     const long long ne = std::max<long long>((long long)A * E, (long long)A + 1);
-    k_uw_edges<<<blocks_of(ne), 256, 0, st>>>(rel, A, H, W, k0, v0, offs);
+    k_uw_edges<R><<<blocks_of(ne), 256, 0, st>>>(rel, phase, A, H, W, k0, v0, offs);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     size_t sb = L.sort_bytes;
     e = rocprim::segmented_radix_sort_pairs(w + L.sort_tmp, sb, k0, k1, v0, v1,
                                             (unsigned)(A * E), (unsigned)A, offs, offs + 1,
                                             0, 64, st);
     if (e != hipSuccess) return e;
-    int *parent = (int *)(w + L.parent), *off = (int *)(w + L.off),
-        *size = (int *)(w + L.size);
-    k_uw_gather<R><<<(A + 63) / 64, 64, 0, st>>>(phase, A, H, W, v1, parent, off, size);
+    int2 *node = (int2 *)(w + L.parent);
+    // one image per wavefront: images advance independently (no lanes
+    // waiting on each other's longer find paths)
+    k_uw_gather<R><<<A, 1, 0, st>>>(A, H, W, v1, node);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_uw_apply<R><<<blocks_of(n), 256, 0, st>>>(phase, A, H, W, parent, off);
+    k_uw_apply<R><<<blocks_of(n), 256, 0, st>>>(phase, A, H, W, node);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   double *remove = nullptr;
