@@ -224,6 +224,33 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Column stage of the 2D path (lNy = Ny = 1, no y-prefilter): the
+// transform along y is the identity, so T[a][k][0][c] = S[a][c] * g_k(c)
+// with g_k as in k_col_ifft.  Threads take consecutive columns (coalesced
+// rows of T), blocks runs of kb slices.
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_prop_2d(const cpx<R> *__restrict__ S, cpx<R> *__restrict__ T,
+              const double *__restrict__ prop_a,
+              const PropAux<R> *__restrict__ aux, const R *__restrict__ q0z,
+              Geom g, int kb, int ldt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k0 = blockIdx.y * kb, ai = blockIdx.z;
+  if (c >= g.Lx) return;
+  const int ln = g.ln;
+  const cpx<R> s = S[(size_t)ai * g.Lx + c];
+  const double pa = prop_a[c];
+  PropAux<R> x{};
+  if (g.fuse_z) x = aux[c];
+  const int kend = min(k0 + kb, ln);
+  for (int k = k0; k < kend; ++k) {
+    const cpx<R> e = phasor(pa * (g.z0 + (double)k), R(0));
+    cpx<R> gk = e;
+    if (g.fuse_z) gk = x.ib * e + q0z[k] * x.r0 + q0z[ln - 1 - k] * x.r1;
+    T[((size_t)ai * ln + k) * ldt + c] = s * gk;
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ void store_val(R *p, cpx<R> v) { *p = v.re; }
 template <typename R>
@@ -1317,6 +1344,12 @@ hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
       case 2048: return col_fft8_launch<R, 2048>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
       case 4096: return col_fft8_launch<R, 4096>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
     }
+  }
+  if (g.lNy == 1 && g.Ny == 1 && g.Ly == 1 && !g.fuse_y) {
+    constexpr int kb = 16;
+    dim3 grid((g.Lx + 255) / 256, (g.ln + kb - 1) / kb, na);
+    k_prop_2d<R><<<grid, 256, 0, st>>>(S, T, pa, aux, q0z, g, kb, ldt);
+    return hipGetLastError();
   }
   const size_t sh = lds_bytes(g.lNy, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_col_ifft<R>, sh);
