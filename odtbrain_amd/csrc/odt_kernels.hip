@@ -1386,10 +1386,13 @@ size_t rotate_window_entries(int nz, int nx) {
   return (size_t)((nz + kRotTile - 1) / kRotTile) * ((nx + kRotTile - 1) / kRotTile);
 }
 
-// Planes per block: 4 when both tile buffers then fit in 64 KiB, else 2.
+// Planes per block: 4 when both tile buffers then fit in 32 KiB, else 2.
+// At 32 KiB per block a CU holds five blocks (20 waves); measured at
+// 256^2 x 360 (complex64): 2 planes 21.1 ms, 3 planes 21.7 ms, 4 planes
+// (64 KiB, two blocks per CU) 22.5 ms, 1 plane 26.3 ms per reconstruction.
 template <typename V>
 constexpr int rot_yc_max() {
-  return 2 * 4 * kRotWin * kRotWin * (int)sizeof(V) <= 65536 ? 4 : 2;
+  return 2 * 4 * kRotWin * kRotWin * (int)sizeof(V) <= 32768 ? 4 : 2;
 }
 
 template <typename R, typename V, int ORDER, int YC>
