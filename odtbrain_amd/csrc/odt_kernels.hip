@@ -668,10 +668,10 @@ __device__ __forceinline__ cpx<R> fma_w(R w, cpx<R> v, cpx<R> acc) {
 // permutation spreads those over the 32 banks for all rotation angles
 // (about 1.7 passes per 8-byte read on average instead of 4).
 // Input coordinates are computed exactly as scipy does (float64, no FMA), so
-// the out-of-bounds set and the tap positions are the reference's; weights,
-// tap sums and the sums over angles run in float64 (the float32 samples are
-// widened exactly); between launches the float32 total carries a
-// compensation term.  Blocks are numbered so that all tiles of one y-chunk
+// the out-of-bounds set and the tap positions are the reference's; weights
+// are formed in float64, the tap sums of a float32 path in float32 (each
+// plane's sum widened exactly) and the sums over angles in float64; between
+// launches the float32 total carries a compensation term.  Blocks are numbered so that all tiles of one y-chunk
 // land on the same XCD (dispatch is round-robin over the 8 XCDs): they share
 // the chunk's planes in that XCD's L2.
 // ---------------------------------------------------------------------------
@@ -867,8 +867,7 @@ struct RotTile {
     const double fx = (ORDER & 1) ? floor(cx) : floor(cx + 0.5);
     const int sz = valid ? (int)fz - ORDER / 2 : wz0 + 1;
     const int sx = valid ? (int)fx - ORDER / 2 : wx0 + 1;
-    // weights and the tap sums in float64 (float64 FMAs issue at the float32
-    // rate on this GPU; the float32 samples are widened exactly)
+    // coordinates, spline start and weights in float64
     double wz[ORDER + 1], wx[ORDER + 1], w[ORDER + 1][ORDER + 1];
     spline_weights_t<ORDER, double>(cz - fz, wz);
     spline_weights_t<ORDER, double>(cx - fx, wx);
@@ -886,18 +885,23 @@ struct RotTile {
       gs[i] = swz(zr[i]) << 4;
       cb[i] = (unsigned)xc[i] * ELB;
     }
+    // tap sums in float32 with the float64 weights rounded once, each
+    // plane's sum widened into the float64 angle sum
+    R wf[ORDER + 1][ORDER + 1];
+#pragma unroll
+    for (int i = 0; i <= ORDER; ++i)
+#pragma unroll
+      for (int j = 0; j <= ORDER; ++j) wf[i][j] = (R)w[i][j];
 #pragma unroll
     for (int q = 0; q < YC; ++q) {
-      wide_t<V> v =
-          mul_w(w[0][0], widen(tap(tile, rb[0] + (cb[0] ^ gs[0]) + q * PLB)));
+      V v = mul_w(wf[0][0], tap(tile, rb[0] + (cb[0] ^ gs[0]) + q * PLB));
 #pragma unroll
       for (int i = 0; i <= ORDER; ++i)
 #pragma unroll
         for (int j = 0; j <= ORDER; ++j)
           if (i | j)
-            v = fma_w(w[i][j],
-                      widen(tap(tile, rb[i] + (cb[j] ^ gs[i]) + q * PLB)), v);
-      acc[q] = acc[q] + v;
+            v = fma_w(wf[i][j], tap(tile, rb[i] + (cb[j] ^ gs[i]) + q * PLB), v);
+      acc[q] = acc[q] + widen(v);
     }
   }
 
