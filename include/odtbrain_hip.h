@@ -209,6 +209,31 @@ int odt_sinogram_prepare(int precision, int ndim, int32_t A, int32_t Ny,
                          int rytov, int align, void *d_out, void *d_ws,
                          size_t ws_bytes, void *stream);
 
+/* Missing-apple-core correction (odtbrain/apple.py:124-262, correct) of an
+ * object function f of shape (n0, n1, n2) with n0 == n2 (the y axis n1 is
+ * the axis of symmetry): iterative projections with the non-negativity
+ * constraint (method 0; method 1, the symmetric-histogram constraint, is
+ * not provided and returns ODT_ERR_UNSUPPORTED), computed in complex128.
+ * d_f complex (f_precision), d_out complex (out_precision; the reference
+ * returns complex64).  d_mask: null or n0*n1*n2 bytes (nonzero = may be
+ * set to zero).  bg_shell_width < 0: no background shell.
+ * enforce_envelope in [0, 1] (0 disables the envelope).  Runs max_iter
+ * iterations at most, stopping as the reference does; the number run is
+ * written to *iterations.  Synchronises the stream once per iteration.
+ * Workspace from odt_apple_workspace_size. */
+int odt_apple_workspace_size(int32_t n0, int32_t n1, int32_t n2, size_t *bytes);
+/* The apple core mask alone (apple.py:7-49, apple_core_3d): d_core gets
+ * n0*n1*n2 bytes, 1 inside the core; n0 == n2 required. */
+int odt_apple_core(int32_t n0, int32_t n1, int32_t n2, double res, double nm,
+                   uint8_t *d_core, void *stream);
+int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2,
+                      const void *d_f, double res, double nm, int method,
+                      const uint8_t *d_mask, double bg_shell_width,
+                      double enforce_envelope, int32_t max_iter,
+                      double min_diff, int out_precision, void *d_out,
+                      void *d_ws, size_t ws_bytes, void *stream,
+                      int32_t *iterations);
+
 const char *odt_last_error(void);
 int odt_version(void);
 

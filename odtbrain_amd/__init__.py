@@ -14,6 +14,7 @@ from ._alg2d_bpp import backpropagate_2d
 from ._alg3d_bpp import backpropagate_3d
 from ._alg3d_bppt import backpropagate_3d_tilted
 from . import util
+from . import apple
 from ._translate_ri import odt_to_ri, opt_to_ri
 from ._prepare_sino import sinogram_as_radon, sinogram_as_rytov
 from .batch import Batch
@@ -23,5 +24,5 @@ __version__ = "0.1.0"
 
 __all__ = ["backpropagate_2d", "backpropagate_3d", "backpropagate_3d_tilted",
            "odt_to_ri", "opt_to_ri", "sinogram_as_radon",
-           "sinogram_as_rytov", "util", "Batch", "backpropagate_3d_adjoint",
+           "sinogram_as_rytov", "util", "apple", "Batch", "backpropagate_3d_adjoint",
            "backpropagate_3d_torch"]

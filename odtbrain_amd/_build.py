@@ -14,7 +14,7 @@ INCLUDE = HERE.parent / "include"
 LIB = HERE / "libodtbrain_hip.so"
 
 SOURCES = ["odt_kernels.hip", "odt_adjoint.hip", "odt_translate.hip",
-           "odt_prepare.hip", "odt_plan.cpp"]
+           "odt_prepare.hip", "odt_apple.hip", "odt_plan.cpp"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 VALU ops.
 HIPCC_FLAGS = [

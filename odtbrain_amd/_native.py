@@ -23,6 +23,7 @@ EXPORTED = (
     "odt_batch_create", "odt_batch_launch", "odt_batch_destroy",
     "odt_odt_to_ri", "odt_opt_to_ri",
     "odt_prepare_workspace_size", "odt_sinogram_prepare",
+    "odt_apple_workspace_size", "odt_apple_core", "odt_apple_correct",
     "odt_last_error", "odt_version",
 )
 
@@ -104,6 +105,15 @@ def lib():
     L.odt_sinogram_prepare.argtypes = [ctypes.c_int, ctypes.c_int, i32, i32,
                                        i32, vp, vp, ctypes.c_int, ctypes.c_int,
                                        vp, vp, ctypes.c_size_t, vp]
+    L.odt_apple_workspace_size.argtypes = [i32, i32, i32,
+                                           ctypes.POINTER(ctypes.c_size_t)]
+    L.odt_apple_core.argtypes = [i32, i32, i32, ctypes.c_double,
+                                 ctypes.c_double, vp, vp]
+    L.odt_apple_correct.argtypes = [
+        ctypes.c_int, i32, i32, i32, vp, ctypes.c_double, ctypes.c_double,
+        ctypes.c_int, vp, ctypes.c_double, ctypes.c_double, i32,
+        ctypes.c_double, ctypes.c_int, vp, vp, ctypes.c_size_t, vp,
+        ctypes.POINTER(i32)]
     L.odt_last_error.restype = ctypes.c_char_p
     L.odt_last_error.argtypes = []
     L.odt_version.argtypes = []

@@ -98,6 +98,25 @@ hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const vo
                           int rytov, int align, void *out, void *ws, size_t ws_bytes,
                           hipStream_t st);
 
+// Missing-apple-core correction (odt_apple.hip), complex128 working arrays.
+hipError_t launch_apple_core(unsigned char *core, int n0, int n1, int n2, double km,
+                             hipStream_t st);
+hipError_t launch_apple_init(const void *f, int f64, cpx<double> *data, long long N,
+                             hipStream_t st);
+hipError_t launch_transpose3(const cpx<double> *in, cpx<double> *out, long long batch,
+                             int rows, int cols, hipStream_t st);
+hipError_t launch_envelope(const cpx<double> *F, const unsigned char *core, int n0, int n1,
+                           int n2, double *hp, double *hp2, unsigned char *cp,
+                           unsigned *nnan_dev, const double *w, int r, double *g, double *g2,
+                           double *env, hipStream_t st);
+hipError_t launch_constraint_nn(cpx<double> *data, int n0, int n1, int n2,
+                                const unsigned char *mask, double bg_width, hipStream_t st);
+hipError_t launch_apple_fourier(cpx<double> *F, cpx<double> *F0, const double *env,
+                                const unsigned char *core, long long N, double factor,
+                                int sh, double *part, double *sum, hipStream_t st);
+hipError_t launch_scale_out(const cpx<double> *in, long long N, double s, int out64,
+                            void *out, hipStream_t st);
+
 // Adjoint stages (odt_adjoint.hip).
 template <typename R>
 hipError_t launch_adj_fir(const cpx<R> *in, cpx<R> *out, long long nlines, int n,

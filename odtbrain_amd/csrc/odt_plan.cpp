@@ -849,6 +849,179 @@ int odt_sinogram_prepare(int precision, int ndim, int32_t A, int32_t Ny, int32_t
   return ODT_OK;
 }
 
+namespace {
+struct AppleLayout {
+  size_t data, ft, ft0, tmp, env, core, hp, hp2, cp, part, sum, nnan, tw0, tw1, tw2, w,
+      total;
+  int r;
+};
+
+AppleLayout apple_layout(int n0, int n1, int n2) {
+  AppleLayout L{};
+  const size_t N = (size_t)n0 * n1 * n2;
+  const int m = std::max(n0, n1);
+  const size_t M = (size_t)m * m * m;
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    const size_t at = o;
+    o = align256(o + b);
+    return at;
+  };
+  L.data = take(N * 16);
+  L.ft = take(N * 16);
+  L.ft0 = take(N * 16);
+  L.tmp = take(N * 16);
+  L.env = take(N * 8);
+  L.core = take(N);
+  L.hp = take(M * 8);
+  L.hp2 = take(M * 8);
+  L.cp = take(M);
+  L.part = take(((N + 255) / 256) * 8);
+  L.sum = take(8);
+  L.nnan = take(4);
+  L.tw0 = take((size_t)n0 * 16);
+  L.tw1 = take((size_t)n1 * 16);
+  L.tw2 = take((size_t)n2 * 16);
+  const double sigma = (double)std::max(n0, std::max(n1, n2)) / 100.0;
+  L.r = (int)(4.0 * sigma + 0.5);
+  L.w = take((size_t)(2 * L.r + 1) * 8);
+  L.total = o;
+  return L;
+}
+
+// 3D transform of a (n0, n1, n2) complex128 array in place: batched 1D
+// transforms along the contiguous axis, the other two axes brought there
+// by batched transposes.
+hipError_t fft3(cpx<double> *a, cpx<double> *tmp, int n0, int n1, int n2, int dir,
+                const cpx<double> *tw0, const cpx<double> *tw1, const cpx<double> *tw2,
+                hipStream_t st) {
+  hipError_t e;
+  auto fft = [&](cpx<double> *x, long long batch, int n, const cpx<double> *tw) {
+    return odt::launch_fft_batch<double>(x, x, batch, n, ilog2_exact(n), dir, tw,
+                                         per_block(n, 16, 16), st);
+  };
+  if ((e = fft(a, (long long)n0 * n1, n2, tw2)) != hipSuccess) return e;
+  if ((e = odt::launch_transpose3(a, tmp, n0, n1, n2, st)) != hipSuccess) return e;
+  if ((e = fft(tmp, (long long)n0 * n2, n1, tw1)) != hipSuccess) return e;
+  if ((e = odt::launch_transpose3(tmp, a, n0, n2, n1, st)) != hipSuccess) return e;
+  if ((e = odt::launch_transpose3(a, tmp, 1, n0, n1 * n2, st)) != hipSuccess) return e;
+  if ((e = fft(tmp, (long long)n1 * n2, n0, tw0)) != hipSuccess) return e;
+  return odt::launch_transpose3(tmp, a, 1, n1 * n2, n0, st);
+}
+}  // namespace
+
+int odt_apple_workspace_size(int32_t n0, int32_t n1, int32_t n2, size_t *bytes) {
+  if (!bytes || n0 < 1 || n1 < 1 || n2 < 1) return fail(ODT_ERR_INVALID, "bad arguments");
+  *bytes = apple_layout(n0, n1, n2).total;
+  return ODT_OK;
+}
+
+int odt_apple_core(int32_t n0, int32_t n1, int32_t n2, double res, double nm,
+                   uint8_t *d_core, void *stream) {
+  if (n0 < 1 || n1 < 1 || n2 < 1 || !d_core) return fail(ODT_ERR_INVALID, "bad arguments");
+  if (n0 != n2) return fail(ODT_ERR_INVALID, "`shape[0]` and `shape[2]` must be identical!");
+  const double km = (2 * M_PI * nm) / res;
+  HIPCHK(odt::launch_apple_core(d_core, n0, n1, n2, km, (hipStream_t)stream), "k_apple_core");
+  return ODT_OK;
+}
+
+int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2, const void *d_f,
+                      double res, double nm, int method, const uint8_t *d_mask,
+                      double bg_shell_width, double enforce_envelope, int32_t max_iter,
+                      double min_diff, int out_precision, void *d_out, void *d_ws,
+                      size_t ws_bytes, void *stream, int32_t *iterations) {
+  if ((f_precision != ODT_FP32 && f_precision != ODT_FP64) ||
+      (out_precision != ODT_FP32 && out_precision != ODT_FP64))
+    return fail(ODT_ERR_INVALID, "precision");
+  if (n0 < 1 || n1 < 1 || n2 < 1 || !d_f || !d_out || !d_ws || max_iter < 1)
+    return fail(ODT_ERR_INVALID, "bad arguments");
+  if (n0 != n2) return fail(ODT_ERR_INVALID, "`shape[0]` and `shape[2]` must be identical!");
+  if (!(enforce_envelope >= 0 && enforce_envelope <= 1))
+    return fail(ODT_ERR_INVALID, "`enforce_envelope` must be in interval [0, 1]");
+  if (method != 0)
+    return fail(ODT_ERR_UNSUPPORTED, "only the non-negativity constraint is provided");
+  const int mmax = std::max(n0, std::max(n1, n2));
+  if (2 * (size_t)mmax * 16 > 160 * 1024 || n0 > 65535)
+    return fail(ODT_ERR_UNSUPPORTED, "volume edge too long for the transforms");
+  const AppleLayout L = apple_layout(n0, n1, n2);
+  if (ws_bytes < L.total)
+    return fail(ODT_ERR_WORKSPACE, "workspace too small: need " + std::to_string(L.total) +
+                                       " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  char *w = (char *)d_ws;
+  cpx<double> *data = (cpx<double> *)(w + L.data), *ft = (cpx<double> *)(w + L.ft),
+              *ft0 = (cpx<double> *)(w + L.ft0), *tmp = (cpx<double> *)(w + L.tmp);
+  double *env = (double *)(w + L.env);
+  unsigned char *core = (unsigned char *)(w + L.core);
+  cpx<double> *tw0 = (cpx<double> *)(w + L.tw0), *tw1 = (cpx<double> *)(w + L.tw1),
+              *tw2 = (cpx<double> *)(w + L.tw2);
+  const long long N = (long long)n0 * n1 * n2;
+  HIPCHK(odt::launch_twiddles<double>(tw0, n0, st), "twiddles");
+  HIPCHK(odt::launch_twiddles<double>(tw1, n1, st), "twiddles");
+  HIPCHK(odt::launch_twiddles<double>(tw2, n2, st), "twiddles");
+  const double km = (2 * M_PI * nm) / res;
+  HIPCHK(odt::launch_apple_core(core, n0, n1, n2, km, st), "k_apple_core");
+  HIPCHK(odt::launch_apple_init(d_f, f_precision == ODT_FP64, data, N, st), "k_apple_init");
+  HIPCHK(hipMemcpyAsync(ft0, data, N * 16, hipMemcpyDeviceToDevice, st), "copy");
+  HIPCHK(fft3(ft0, tmp, n0, n1, n2, -1, tw0, tw1, tw2, st), "fft3");
+  const bool use_env = enforce_envelope != 0.0;
+  if (use_env) {
+    // scipy.ndimage gaussian weights: exp(-x^2 / (2 sigma^2)) normalised
+    const double sigma = (double)mmax / 100.0;
+    std::vector<double> wt(2 * L.r + 1);
+    double ssum = 0;
+    for (int k = -L.r; k <= L.r; ++k) {
+      wt[k + L.r] = std::exp(-0.5 / (sigma * sigma) * (double)k * (double)k);
+      ssum += wt[k + L.r];
+    }
+    for (auto &v : wt) v /= ssum;
+    HIPCHK(hipMemcpy(w + L.w, wt.data(), wt.size() * 8, hipMemcpyHostToDevice), "copy w");
+    HIPCHK(hipMemsetAsync(w + L.nnan, 0, 4, st), "memset");
+    double *g = (double *)tmp, *g2 = (double *)tmp + N;
+    HIPCHK(odt::launch_envelope(ft0, core, n0, n1, n2, (double *)(w + L.hp),
+                                (double *)(w + L.hp2), (unsigned char *)(w + L.cp),
+                                (unsigned *)(w + L.nnan), (const double *)(w + L.w), L.r, g,
+                                g2, env, st),
+           "envelope");
+  }
+  // initial state: sum |F0| over the core
+  HIPCHK(hipMemcpyAsync(ft, ft0, N * 16, hipMemcpyDeviceToDevice, st), "copy");
+  HIPCHK(odt::launch_apple_fourier(ft, ft0, nullptr, core, N, 1.0, 0,
+                                   (double *)(w + L.part), (double *)(w + L.sum), st),
+         "k_apple_fourier");
+  double state = 0;
+  HIPCHK(hipMemcpyAsync(&state, w + L.sum, 8, hipMemcpyDeviceToHost, st), "copy");
+  HIPCHK(hipStreamSynchronize(st), "sync");
+  double prev = state / (double)N, norm = 0;
+  int ii = 0;
+  for (; ii < max_iter; ++ii) {
+    HIPCHK(odt::launch_constraint_nn(data, n0, n1, n2, d_mask, bg_shell_width, st),
+           "k_constraint_nn");
+    HIPCHK(hipMemcpyAsync(ft, data, N * 16, hipMemcpyDeviceToDevice, st), "copy");
+    HIPCHK(fft3(ft, tmp, n0, n1, n2, -1, tw0, tw1, tw2, st), "fft3");
+    HIPCHK(odt::launch_apple_fourier(ft, ft0, use_env ? env : nullptr, core, N,
+                                     enforce_envelope, 0, (double *)(w + L.part),
+                                     (double *)(w + L.sum), st),
+           "k_apple_fourier");
+    HIPCHK(hipMemcpyAsync(data, ft, N * 16, hipMemcpyDeviceToDevice, st), "copy");
+    HIPCHK(fft3(data, tmp, n0, n1, n2, +1, tw0, tw1, tw2, st), "fft3");
+    HIPCHK(odt::launch_scale_out(data, N, 1.0 / (double)N, 1, data, st), "scale");
+    HIPCHK(hipMemcpyAsync(&state, w + L.sum, 8, hipMemcpyDeviceToHost, st), "copy");
+    HIPCHK(hipStreamSynchronize(st), "sync");
+    const double cur = state / (double)N;
+    const double diff = cur - prev;
+    if (ii == 0) {
+      norm = diff;
+    } else if (std::fabs(diff / norm) < min_diff) {
+      break;
+    }
+    prev = cur;
+  }
+  if (iterations) *iterations = std::min(ii + 1, (int)max_iter);
+  HIPCHK(odt::launch_scale_out(data, N, 1.0, out_precision == ODT_FP64, d_out, st), "out");
+  return ODT_OK;
+}
+
 int odt_spline_prefilter(int precision, int order, int64_t nlines, int32_t n,
                          int64_t line_stride, int64_t elem_stride, void *d_data,
                          void *stream) {

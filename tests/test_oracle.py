@@ -302,3 +302,44 @@ def test_align_and_divmod_neg_known_answers():
     sino = base + jumps + 3 * twopi
     P.align_unwrapped(sino)
     assert np.allclose(sino, base)
+
+
+# --------------------------------------------------------------------------
+# apple.correct (odtbrain/apple.py)
+# --------------------------------------------------------------------------
+def test_apple_oracle_fixture():
+    """tests/test_apple.py:48-69 through the oracle's own preparation and
+    reconstruction (atol 2e-6, the reference test's bar)."""
+    from oracle import apple as AP
+    from oracle import prepare as P
+    from oracle import odt_oracle as O
+    sino, angles = create_test_sino_3d(Nx=10, Ny=10)
+    p = get_test_parameter_set(1)[0]
+    sryt = P.sinogram_as_rytov(uSin=sino, u0=1, align=False)
+    f = O.backpropagate_3d(sryt, angles, padval=0, dtype=float, **p)
+    fc = AP.correct(f=f, res=p["res"], nm=p["nm"], enforce_envelope=.95,
+                    max_iter=100, min_diff=0.01)
+    fo = np.array(cutout(fc), dtype=np.complex128)
+    assert np.allclose(fo.flatten().view(float),
+                       load_golden("apple__test_correct_reproduce"),
+                       atol=2e-6)
+
+
+def test_apple_masks_closed_forms():
+    """apple_core_3d: the axis of symmetry is y; on the ky axis (kx = kz =
+    0) every frequency with |ky| > 0 inside the sqrt(2) km sphere is core,
+    nothing outside it; spillover_region is the complement of the inscribed
+    ellipsoid (centre excluded, corners included)."""
+    from oracle import apple as AP
+    shape = (16, 12, 16)
+    res, nm = 3.0, 1.333
+    core = AP.apple_core_3d(shape, res, nm)
+    km = 2 * np.pi * nm / res
+    ky = 2 * np.pi * np.fft.fftfreq(12)
+    expect = (np.abs(ky) > 0) & (np.abs(ky) <= np.sqrt(2) * km)
+    assert np.array_equal(core[0, :, 0], expect)
+    assert not core[0, 0, 0]
+    with pytest.raises(ValueError):
+        AP.apple_core_3d((10, 10, 5), .1, 1)
+    spov = AP.spillover_region((8, 6, 8))
+    assert not spov[4, 3, 4] and spov[0, 0, 0] and spov[7, 5, 7]
