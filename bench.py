@@ -238,6 +238,38 @@ def adjoint_bench(steps, warmup, directions=64):
     print(json.dumps(line), flush=True)
 
 
+def apple_bench(max_iter):
+    """apple.correct (non-negativity, envelope 0.95) of the 256^3 FP32
+    reconstruction of the 256^2 x 360 workload, ``max_iter`` iterations
+    (min_diff 0 so that all run); time per call and per iteration."""
+    import torch
+    import odtbrain_amd as G
+    from odtbrain_amd import phantoms
+    dev = torch.device("cuda:0")
+    uB, angles = phantoms.ellipsoid_sinogram_3d(N=N_DET, A=N_ANG)
+    f = G.backpropagate_3d(torch.from_numpy(uB).to(dev), angles, RES, NM,
+                           dtype=np.float32)
+    G.apple.correct(f, RES, NM, max_iter=2, min_diff=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = G.apple.correct(f, RES, NM, max_iter=max_iter, min_diff=0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    line = {
+        "metric": "apple-core correction iterations/s (256^3 volume)",
+        "value": max_iter / dt, "unit": "iterations/s", "n_gpus": 1,
+        "steps": max_iter, "warmup": 2, "ms_per_step": dt / max_iter * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (FP32 reconstruction of the "
+                                "3-ellipsoid 256^2 x 360 sinogram)",
+        "config": {"workload": "apple.correct method nn, enforce_envelope "
+                               "0.95, 256^3 complex, transforms in complex128"},
+        "total_ms": dt * 1e3,
+        "finite": bool(torch.isfinite(out.real).all()),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +283,9 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="run the differentiable (gradient-descent) workload "
                          "instead; --steps defaults to 50 there")
+    ap.add_argument("--apple", action="store_true",
+                    help="run apple.correct on the 256^3 reconstruction "
+                         "instead (--steps iterations, default 100)")
     ap.add_argument("--fd-directions", type=int, default=64,
                     help="finite-difference directions of --adjoint")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -265,6 +300,9 @@ def main():
 
     if args.batch:
         batch_bench(args.steps, args.warmup)
+        return
+    if args.apple:
+        apple_bench(100 if args.steps == 5 else args.steps)
         return
     if args.adjoint:
         adjoint_bench(50 if args.steps == 5 else args.steps, args.warmup,
