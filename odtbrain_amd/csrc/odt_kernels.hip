@@ -163,6 +163,40 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Propagator table g_k(cx, cy) of every slice k and support point: the
+// propagator exp(i a z_k), a = km (M - 1) from the float64 table, evaluated
+// with float64 sin/cos, times the fused z-prefilter terms where present
+// (see k_col_ifft), rounded once.  The same for every angle, so the column
+// stage reads it instead of evaluating it per angle.
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_prop_table(const double *__restrict__ prop_a, const PropAux<R> *__restrict__ aux,
+                 const R *__restrict__ q0z, Geom g, cpx<R> *__restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = (long long)g.Lx * g.Ly;
+  if (i >= per * g.ln) return;
+  const int k = (int)(i / per);
+  const long long t = i - (long long)k * per;
+  double sn, cs;
+  sincos(prop_a[t] * (g.z0 + (double)k), &sn, &cs);
+  double re = cs, im = sn;
+  if (g.fuse_z) {
+    const PropAux<R> x = aux[t];
+    const double ib = (double)x.ib, qa = (double)q0z[k], qb = (double)q0z[g.ln - 1 - k];
+    re = ib * cs + qa * (double)x.r0.re + qb * (double)x.r1.re;
+    im = ib * sn + qa * (double)x.r0.im + qb * (double)x.r1.im;
+  }
+  out[i] = mk<R>((R)re, (R)im);
+}
+
+template <typename R>
+hipError_t launch_prop_table(const double *pa, const PropAux<R> *aux, const R *q0z,
+                             const Geom &g, cpx<R> *out, hipStream_t st) {
+  const long long n = (long long)g.Lx * g.Ly * g.ln;
+  k_prop_table<R><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(pa, aux, q0z, g, out);
+  return hipGetLastError();
+}
+
 // Column stage: for x-frequency column c and slices k0..k0+nb-1 build
 // S*g_k along the y-frequencies (zero outside the support), inverse
 // transform along y, crop rows padyl..padyl+Ny-1 (plus the fused
@@ -174,7 +208,7 @@ __global__ void __launch_bounds__(256)
                const double *__restrict__ prop_a,
                const PropAux<R> *__restrict__ aux,
                const R *__restrict__ q0z, const R *__restrict__ q0y, Geom g,
-               int nb, int ldt) {
+               int nb, int ldt, const cpx<R> *__restrict__ gtab) {
   cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
   cpx<R> *tmp = buf + nb * g.lNy;
   __shared__ cpx<R> corr[2 * 16];
@@ -191,11 +225,16 @@ __global__ void __launch_bounds__(256)
     const int k = k0 + r;
     if (k >= ln) continue;
     const size_t t = (size_t)c * g.Ly + cy;
-    const cpx<R> e = phasor(prop_a[t] * (g.z0 + (double)k), R(0));
-    cpx<R> gk = e;
-    if (g.fuse_z) {
-      const PropAux<R> x = aux[t];
-      gk = x.ib * e + q0z[k] * x.r0 + q0z[ln - 1 - k] * x.r1;
+    cpx<R> gk;
+    if (gtab) {
+      gk = gtab[(size_t)k * g.Lx * g.Ly + t];
+    } else {
+      const cpx<R> e = phasor(prop_a[t] * (g.z0 + (double)k), R(0));
+      gk = e;
+      if (g.fuse_z) {
+        const PropAux<R> x = aux[t];
+        gk = x.ib * e + q0z[k] * x.r0 + q0z[ln - 1 - k] * x.r1;
+      }
     }
     buf[r * g.lNy + wrap(g.jy0 + cy, g.lNy)] = Sc[cy] * gk;
   }
@@ -233,7 +272,7 @@ __global__ void __launch_bounds__(256)
     k_prop_2d(const cpx<R> *__restrict__ S, cpx<R> *__restrict__ T,
               const double *__restrict__ prop_a,
               const PropAux<R> *__restrict__ aux, const R *__restrict__ q0z,
-              Geom g, int kb, int ldt) {
+              Geom g, int kb, int ldt, const cpx<R> *__restrict__ gtab) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int k0 = blockIdx.y * kb, ai = blockIdx.z;
   if (c >= g.Lx) return;
@@ -244,9 +283,14 @@ __global__ void __launch_bounds__(256)
   if (g.fuse_z) x = aux[c];
   const int kend = min(k0 + kb, ln);
   for (int k = k0; k < kend; ++k) {
-    const cpx<R> e = phasor(pa * (g.z0 + (double)k), R(0));
-    cpx<R> gk = e;
-    if (g.fuse_z) gk = x.ib * e + q0z[k] * x.r0 + q0z[ln - 1 - k] * x.r1;
+    cpx<R> gk;
+    if (gtab) {
+      gk = gtab[(size_t)k * g.Lx + c];
+    } else {
+      const cpx<R> e = phasor(pa * (g.z0 + (double)k), R(0));
+      gk = e;
+      if (g.fuse_z) gk = x.ib * e + q0z[k] * x.r0 + q0z[ln - 1 - k] * x.r1;
+    }
     T[((size_t)ai * ln + k) * ldt + c] = s * gk;
   }
 }
@@ -378,7 +422,7 @@ __global__ void __launch_bounds__(512)
                const double *__restrict__ prop_a,
                const PropAux<R> *__restrict__ aux,
                const R *__restrict__ q0z, const R *__restrict__ q0y, Geom g,
-               int kb, int ldt, int staged) {
+               int kb, int ldt, int staged, const cpx<R> *__restrict__ gtab) {
   using FFT = Fft8<R, N, +1>;
   constexpr int TT = FFT::T;
   constexpr bool WAVE = TT <= 64;
@@ -424,11 +468,16 @@ __global__ void __launch_bounds__(512)
       for (int m = 0; m < 8; ++m) {
         if (!slot_used<N, BAND>(m)) continue;
         const int cy = cyc[m];
-        cpx<R> gk = phasor(prop_a[tr + cy] * z, R(0));
-        if (g.fuse_z) {
-          const PropAux<R> *x = aux + tr + cy;
-          gk = x->ib * gk;
-          if (edge) gk = gk + qa * x->r0 + qb * x->r1;
+        cpx<R> gk;
+        if (gtab) {
+          gk = gtab[(size_t)k * g.Lx * g.Ly + tr + cy];
+        } else {
+          gk = phasor(prop_a[tr + cy] * z, R(0));
+          if (g.fuse_z) {
+            const PropAux<R> *x = aux + tr + cy;
+            gk = x->ib * gk;
+            if (edge) gk = gk + qa * x->r0 + qb * x->r1;
+          }
         }
         const cpx<R> val = Sc[cy] * gk;
         const bool ok = live && cyv[m];
@@ -1261,7 +1310,7 @@ static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
                                   const cpx<R> *twy, const double *pa,
                                   const PropAux<R> *aux, const R *q0z,
                                   const R *q0y, const Geom &g, int na, int ldt,
-                                  hipStream_t st) {
+                                  const cpx<R> *gtab, hipStream_t st) {
   constexpr int TT = N / 8;
   // 256-thread blocks: at ~150 VGPRs a SIMD holds three waves, so three
   // 4-wave blocks fit per CU where only one 8-wave block would
@@ -1282,10 +1331,10 @@ static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
   if (e != hipSuccess) return e;
   if (band)
     k_col_fft8<R, N, true><<<grid, tpb * TT, sh, st>>>(S, T, twy, pa, aux, q0z,
-                                                       q0y, g, kb, ldt, staged);
+                                                       q0y, g, kb, ldt, staged, gtab);
   else
     k_col_fft8<R, N, false><<<grid, tpb * TT, sh, st>>>(S, T, twy, pa, aux, q0z,
-                                                        q0y, g, kb, ldt, staged);
+                                                        q0y, g, kb, ldt, staged, gtab);
   return hipGetLastError();
 }
 
@@ -1337,29 +1386,30 @@ template <typename R>
 hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
                            const double *pa, const PropAux<R> *aux,
                            const R *q0z, const R *q0y, const Geom &g, int na,
-                           int nb, int ldt, hipStream_t st) {
+                           int nb, int ldt, const cpx<R> *gtab,
+                           hipStream_t st) {
   if (fft8_col_ok<R>(g)) {
     switch (g.lNy) {
-      case 64: return col_fft8_launch<R, 64>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
-      case 128: return col_fft8_launch<R, 128>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
-      case 256: return col_fft8_launch<R, 256>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
-      case 512: return col_fft8_launch<R, 512>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
-      case 1024: return col_fft8_launch<R, 1024>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
-      case 2048: return col_fft8_launch<R, 2048>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
-      case 4096: return col_fft8_launch<R, 4096>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, st);
+      case 64: return col_fft8_launch<R, 64>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, gtab, st);
+      case 128: return col_fft8_launch<R, 128>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, gtab, st);
+      case 256: return col_fft8_launch<R, 256>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, gtab, st);
+      case 512: return col_fft8_launch<R, 512>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, gtab, st);
+      case 1024: return col_fft8_launch<R, 1024>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, gtab, st);
+      case 2048: return col_fft8_launch<R, 2048>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, gtab, st);
+      case 4096: return col_fft8_launch<R, 4096>(S, T, twy, pa, aux, q0z, q0y, g, na, ldt, gtab, st);
     }
   }
   if (g.lNy == 1 && g.Ny == 1 && g.Ly == 1 && !g.fuse_y) {
     constexpr int kb = 16;
     dim3 grid((g.Lx + 255) / 256, (g.ln + kb - 1) / kb, na);
-    k_prop_2d<R><<<grid, 256, 0, st>>>(S, T, pa, aux, q0z, g, kb, ldt);
+    k_prop_2d<R><<<grid, 256, 0, st>>>(S, T, pa, aux, q0z, g, kb, ldt, gtab);
     return hipGetLastError();
   }
   const size_t sh = lds_bytes(g.lNy, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_col_ifft<R>, sh);
   if (e != hipSuccess) return e;
   dim3 grid(g.Lx, (g.ln + nb - 1) / nb, na);
-  k_col_ifft<R><<<grid, 256, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g, nb, ldt);
+  k_col_ifft<R><<<grid, 256, sh, st>>>(S, T, twy, pa, aux, q0z, q0y, g, nb, ldt, gtab);
   return hipGetLastError();
 }
 
@@ -1612,7 +1662,10 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
   template hipError_t launch_col_ifft<R>(                                      \
       const cpx<R> *, cpx<R> *, const cpx<R> *, const double *,                \
       const PropAux<R> *, const R *, const R *, const Geom &, int, int, int,   \
-      hipStream_t);                                                            \
+      const cpx<R> *, hipStream_t);                                            \
+  template hipError_t launch_prop_table<R>(const double *, const PropAux<R> *, \
+                                           const R *, const Geom &, cpx<R> *,  \
+                                           hipStream_t);                                                            \
   template hipError_t launch_row_ifft<R, R>(const cpx<R> *, R *,               \
                                             const cpx<R> *, const R *,         \
                                             const Geom &, int, int, int, int,  \

@@ -44,7 +44,11 @@ template <typename R>
 hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
                            const double *pa, const PropAux<R> *aux,
                            const R *q0z, const R *q0y, const Geom &g, int na,
-                           int nb, int ldt, hipStream_t st);
+                           int nb, int ldt, const cpx<R> *gtab, hipStream_t st);
+// Propagator table (ln x Lx x Ly) for the column stage.
+template <typename R>
+hipError_t launch_prop_table(const double *pa, const PropAux<R> *aux, const R *q0z,
+                             const Geom &g, cpx<R> *out, hipStream_t st);
 template <typename R, typename V>
 hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
                            const R *q0x, const Geom &g, int na, int nb,

@@ -129,6 +129,7 @@ struct odt_plan {
   void *twx = nullptr, *twy = nullptr, *P = nullptr, *aux = nullptr;
   void *q0x = nullptr, *q0y = nullptr, *q0z = nullptr;
   double *prop_a = nullptr, *weights = nullptr, *params = nullptr;
+  void *gtab = nullptr;      // propagator table of the column stage (or null)
   void *fir_taps = nullptr;  // adjoint: prefilter impulse response (lazily)
   int fir_K = 0;
   // timing
@@ -220,6 +221,16 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   HIPCHK(hipMalloc(&p->twy, (size_t)g.lNy * sizeof(cpx<R>)), "hipMalloc(twy)");
   HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twx, g.lNx, 0), "twiddles x");
   HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twy, g.lNy, 0), "twiddles y");
+  // The column stage's propagator (times the fused z-prefilter terms) is
+  // the same for every angle: tabulated once per plan when it fits in
+  // 2 GiB (C2: 256 x 227 x 227 complex64, 106 MB).
+  const size_t tab = (size_t)g.ln * g.Lx * g.Ly * sizeof(cpx<R>);
+  if (tab <= ((size_t)2 << 30) && !std::getenv("ODTBRAIN_AMD_NO_PROP_TABLE")) {
+    HIPCHK(hipMalloc(&p->gtab, tab), "hipMalloc(gtab)");
+    HIPCHK(odt::launch_prop_table<R>(p->prop_a, (const PropAux<R> *)p->aux,
+                                     (const R *)p->q0z, g, (cpx<R> *)p->gtab, 0),
+           "k_prop_table");
+  }
   HIPCHK(hipDeviceSynchronize(), "plan tables");
   return ODT_OK;
 }
@@ -233,7 +244,7 @@ int per_block(int n, size_t esz, int cap) {
 
 void free_plan(odt_plan *p) {
   void *ptrs[] = {p->twx, p->twy, p->P, p->aux, p->q0x, p->q0y, p->q0z,
-                  p->prop_a, p->weights, p->params, p->fir_taps};
+                  p->prop_a, p->weights, p->params, p->fir_taps, p->gtab};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto e : p->ev_rot) (void)hipEventDestroy(e);
@@ -269,7 +280,8 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
              "k_fwd_cols");
       HIPCHK(odt::launch_col_ifft<R>(S, T, (const cpx<R> *)p->twy, p->prop_a,
                                      (const PropAux<R> *)p->aux, (const R *)p->q0z,
-                                     (const R *)p->q0y, g, n, p->nb_col, p->ldt, st),
+                                     (const R *)p->q0y, g, n, p->nb_col, p->ldt,
+                                     (const cpx<R> *)p->gtab, st),
              "k_col_ifft");
       HIPCHK((odt::launch_row_ifft<R, V>(T, slab, (const cpx<R> *)p->twx,
                                          (const R *)p->q0x, g, n, p->nb_row, c0 - b0,
