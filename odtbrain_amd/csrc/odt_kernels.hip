@@ -763,14 +763,34 @@ __device__ __forceinline__ int rot_origin(double lo, double hi, int n, int al) {
   return w < wmax ? w : wmax;
 }
 
-// Window origins (z, x) of every (angle, tile).  A tile's computed input
+// Rows [lo, hi] (window coordinates, inclusive, clipped to the window) that
+// hold every tap of an in-range coordinate of [clo, chi], mirrored taps
+// included: from one below the lowest spline start to the highest start
+// plus ORDER, within the plane (mirrored taps land inside that range).
+// Empty (lo > hi) when no coordinate is in range.
+template <int ORDER>
+__device__ __forceinline__ int rot_extent(double clo, double chi, int n, int w0) {
+  clo = fmax(clo, 0.0);
+  chi = fmin(chi, (double)(n - 1));
+  if (!(clo <= chi)) return 1;  // lo = 1, hi = 0
+  int lo = spline_start<ORDER>(clo) - 1, hi = spline_start<ORDER>(chi) + ORDER;
+  lo = (lo < 0 ? 0 : lo) - w0;
+  hi = (hi > n - 1 ? n - 1 : hi) - w0;
+  lo = lo < 0 ? 0 : lo;
+  hi = hi > kRotWin - 1 ? kRotWin - 1 : hi;
+  return lo | (hi << 8);
+}
+
+// Window origins (z, x) of every (angle, tile) and the extents of the
+// window the tile's taps use (packed rows lo | hi << 8 | cols lo << 16 |
+// hi << 24): the copy into LDS fetches only those.  A tile's computed input
 // coordinates are monotone in each output index (every rounding step is), so
 // the four corners, evaluated with the voxels' own float64 operations, span
 // them exactly.
 template <int ORDER>
 __global__ void k_rot_windows(const double *__restrict__ params, int nb, int ntz,
                               int ntx, int nz, int nx, int al,
-                              int2 *__restrict__ win) {
+                              int4 *__restrict__ win) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int nt = ntz * ntx;
   if (i >= (long long)nb * nt) return;
@@ -790,8 +810,11 @@ __global__ void k_rot_windows(const double *__restrict__ params, int nb, int ntz
     xlo = c ? fmin(xlo, cx) : cx;
     xhi = c ? fmax(xhi, cx) : cx;
   }
-  win[i] = make_int2(rot_origin<ORDER>(zlo, zhi, nz, 1),
-                     rot_origin<ORDER>(xlo, xhi, nx, al));
+  const int wz0 = rot_origin<ORDER>(zlo, zhi, nz, 1);
+  const int wx0 = rot_origin<ORDER>(xlo, xhi, nx, al);
+  const int ez = rot_extent<ORDER>(zlo, zhi, nz, wz0);
+  const int ex = rot_extent<ORDER>(xlo, xhi, nx, wx0);
+  win[i] = make_int4(wz0, wx0, ez | (ex << 16), 0);
 }
 
 template <typename R>
@@ -847,13 +870,21 @@ struct RotTile {
   // them lets the compiler merge their last loads, which loses the LDS
   // object of the merged load and makes every later LDS read wait for all
   // loads in flight.
+  // Slots outside the window extent `ext` (see k_rot_windows) load the
+  // block's first sample(s) instead (one cache line for the whole wave): every
+  // load instruction is still issued, so the vmcnt waits keep counting K
+  // per wave, and every slot holds a finite value for the zero-weight reads
+  // of out-of-range voxels.
   __device__ static __forceinline__ void issue(const V *coefA, int wz0, int wx0,
-                                               char *dst, size_t plane, int nyv,
-                                               int nz, int nx, int wave,
+                                               int ext, char *dst, size_t plane,
+                                               int nyv, int nz, int nx, int wave,
                                                int lane, const unsigned *lo) {
     const char *src = reinterpret_cast<const char *>(coefA);
+    const int rlo = ext & 0xff, rhi = (ext >> 8) & 0xff;
+    const int clo = (ext >> 16) & 0xff, chi = (ext >> 24) & 0xff;
     if constexpr (VEC == 4) {
       const unsigned uo = (unsigned)(wz0 * nx + wx0) * ELB;
+      const int rl = lane / NG, gl = lane % NG;
 #pragma unroll
       for (int i = 0; i < K; ++i) {
         const int k = wave * K + i;
@@ -861,8 +892,12 @@ struct RotTile {
         const int q = rt / kRotWin, r = rt % kRotWin;
         const int qq = q < nyv ? q : 0;
         const char *g = src + ((size_t)qq * plane + (size_t)r * nx) * ELB;
-        __builtin_amdgcn_global_load_lds((gbl_vptr)(g + (lo[i % NLO] + uo)),
-                                         (lds_vptr)(dst + k * BPI), 16, 0, 0);
+        const int row = r + rl;
+        const int c0 = (int)(gl ^ swz(rt + rl)) * AL;
+        const bool need = row >= rlo && row <= rhi && c0 <= chi && c0 + AL - 1 >= clo;
+        const char *a = need ? g + (lo[i % NLO] + uo) : src;
+        __builtin_amdgcn_global_load_lds((gbl_vptr)a, (lds_vptr)(dst + k * BPI), 16,
+                                         0, 0);
       }
     } else {
 #pragma unroll
@@ -878,8 +913,12 @@ struct RotTile {
         int col = wx0 + e;
         col = col < nx - 1 ? col : nx - 1;
         const int qq = q < nyv ? q : 0;
+        const bool need = r >= rlo && r <= rhi && e >= clo && e <= chi;
+        // the stand-in keeps the byte position within the sample, so a
+        // skipped slot holds the bytes of one whole (finite) sample
         const char *g =
-            src + ((size_t)qq * plane + (size_t)row * nx + col) * ELB + eb;
+            need ? src + ((size_t)qq * plane + (size_t)row * nx + col) * ELB + eb
+                 : src + eb;
         __builtin_amdgcn_global_load_lds((gbl_vptr)g, (lds_vptr)(dst + k * BPI),
                                          4, 0, 0);
       }
@@ -988,7 +1027,7 @@ template <typename R, typename V, int ORDER, int YC, int VEC>
 __global__ void __launch_bounds__(256)
     k_rotate_lds(const V *__restrict__ coef, int nb,
                  const double *__restrict__ params,
-                 const int2 *__restrict__ win, V *__restrict__ out,
+                 const int4 *__restrict__ win, V *__restrict__ out,
                  V *__restrict__ comp, int ny, int nz, int nx, int first,
                  int last, int ntx, int ntz, int xcd_map) {
   using RT = RotTile<R, V, ORDER, YC, VEC>;
@@ -1029,8 +1068,8 @@ __global__ void __launch_bounds__(256)
   const double oz = (double)z, ox = (double)x;
   const V *base = coef + (size_t)y0 * plane;
   const size_t aStride = (size_t)ny * plane;
-  const int2 *wt = win + ti;
-  int2 wA = make_int2(0, 0), wB = make_int2(0, 0);
+  const int4 *wt = win + ti;
+  int4 wA = make_int4(0, 0, 1, 0), wB = make_int4(0, 0, 1, 0);
   // Software pipeline over angles, two per iteration, every prefetch issued
   // from the loop body: angle a+1 goes to B while a (in A) is computed, then
   // a+2 goes to A while a+1 (in B) is computed.  The first iteration starts
@@ -1038,8 +1077,8 @@ __global__ void __launch_bounds__(256)
   for (int a = -1; a < nb; a += 2) {
     if (a + 1 < nb) {
       wB = wt[(size_t)(a + 1) * nt];
-      RT::issue(base + (a + 1) * aStride, wB.x, wB.y, bufB, plane, nyv, nz, nx,
-                wave, lane, lo);
+      RT::issue(base + (a + 1) * aStride, wB.x, wB.y, wB.z, bufB, plane, nyv, nz,
+                nx, wave, lane, lo);
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
     } else {
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
@@ -1051,8 +1090,8 @@ __global__ void __launch_bounds__(256)
     if (a + 1 < nb) {
       if (a + 2 < nb) {
         wA = wt[(size_t)(a + 2) * nt];
-        RT::issue(base + (a + 2) * aStride, wA.x, wA.y, bufA, plane, nyv, nz,
-                  nx, wave, lane, lo);
+        RT::issue(base + (a + 2) * aStride, wA.x, wA.y, wA.z, bufA, plane, nyv,
+                  nz, nx, wave, lane, lo);
         __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
       } else {
         __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
@@ -1479,7 +1518,7 @@ static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
                     ((uintptr_t)coef & 15) == 0;
   const long long nw = (long long)nb * nt;
   k_rot_windows<ORDER><<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(
-      params, nb, ntz, ntx, nz, nx, vec4 ? AL : 1, (int2 *)win);
+      params, nb, ntz, ntx, nz, nx, vec4 ? AL : 1, (int4 *)win);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int nyc = (ny + YC - 1) / YC;
@@ -1488,11 +1527,11 @@ static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
       xcd_map ? 8LL * nt * ((nyc + 7) / 8) : (long long)nt * nyc;
   if (vec4)
     k_rotate_lds<R, V, ORDER, YC, 4><<<(unsigned)blocks, 256, 0, st>>>(
-        coef, nb, params, (const int2 *)win, out, comp, ny, nz, nx, first, last,
+        coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
         ntx, ntz, xcd_map);
   else
     k_rotate_lds<R, V, ORDER, YC, 1><<<(unsigned)blocks, 256, 0, st>>>(
-        coef, nb, params, (const int2 *)win, out, comp, ny, nz, nx, first, last,
+        coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
         ntx, ntz, xcd_map);
   return hipGetLastError();
 }

@@ -54,7 +54,7 @@ hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
                            const R *q0x, const Geom &g, int na, int nb,
                            int slab0, int ldt, hipStream_t st);
 // Rotate-accumulate of nb angles; win is scratch of
-// 2 * nb * rotate_window_entries(nz, nx) ints (window origins per tile).
+// 4 * nb * rotate_window_entries(nz, nx) ints (window origin and extents per tile).
 size_t rotate_window_entries(int nz, int nx);
 template <typename R, typename V>
 hipError_t launch_rotate_acc(int order, const V *coef, int nb,

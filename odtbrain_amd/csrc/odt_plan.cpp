@@ -572,9 +572,9 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
     p->off_comp = off;
     off = align256(off + slab_angle);
   }
-  // window origins of the rotate-accumulate tiles (two ints per tile/angle)
+  // windows of the rotate-accumulate tiles (four ints per tile/angle)
   p->off_win = off;
-  off = align256(off + 2 * sizeof(int) * (size_t)p->NB *
+  off = align256(off + 4 * sizeof(int) * (size_t)p->NB *
                            odt::rotate_window_entries(p->g.ln, p->g.Nx));
   p->ws_bytes = off;
 
@@ -1074,7 +1074,7 @@ int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
   hipStream_t st = (hipStream_t)stream;
   // parameters followed by the window-origin scratch of the tiles
   const size_t pbytes = align256(sizeof(double) * 6 * nang);
-  const size_t wbytes = 2 * sizeof(int) * (size_t)nang * odt::rotate_window_entries(nz, nx);
+  const size_t wbytes = 4 * sizeof(int) * (size_t)nang * odt::rotate_window_entries(nz, nx);
   char *dp = nullptr;
   HIPCHK(hipMalloc((void **)&dp, pbytes + wbytes), "hipMalloc(params)");
   hipError_t e = hipMemcpyAsync(dp, params, sizeof(double) * 6 * nang,
