@@ -728,10 +728,11 @@ __device__ __forceinline__ cpx<R> fma_w(R w, cpx<R> v, cpx<R> acc) {
 // permutation spreads those over the 32 banks for all rotation angles
 // (about 1.7 passes per 8-byte read on average instead of 4).
 // Input coordinates are computed exactly as scipy does (float64, no FMA), so
-// the out-of-bounds set and the tap positions are the reference's; weights
-// are formed in float64, the tap sums of a float32 path in float32 (each
-// plane's sum widened exactly) and the sums over angles in float64; between
-// launches the float32 total carries a compensation term.  Blocks are numbered so that all tiles of one y-chunk
+// the out-of-bounds set and the tap positions are the reference's; the
+// weights and tap sums of a float32 path are formed in float32 from the exact
+// fractional offsets (each plane's sum widened exactly) and the sums over
+// angles in float64; between launches the float32 total carries a
+// compensation term.  Blocks are numbered so that all tiles of one y-chunk
 // land on the same XCD (dispatch is round-robin over the 8 XCDs): they share
 // the chunk's planes in that XCD's L2.
 // ---------------------------------------------------------------------------
@@ -966,14 +967,26 @@ struct RotTile {
     const double fx = (ORDER & 1) ? floor(cx) : floor(cx + 0.5);
     const int sz = valid ? (int)fz - ORDER / 2 : wz0 + 1;
     const int sx = valid ? (int)fx - ORDER / 2 : wx0 + 1;
-    // coordinates, spline start and weights in float64
-    double wz[ORDER + 1], wx[ORDER + 1], w[ORDER + 1][ORDER + 1];
-    spline_weights_t<ORDER, double>(cz - fz, wz);
-    spline_weights_t<ORDER, double>(cx - fx, wx);
+    // coordinates and spline start in float64; the weights of the float32
+    // path in float32 from the exact fractional offsets, their products
+    // formed once for both planes
+    double wzd[ORDER + 1], wxd[ORDER + 1];
+    R wzr[ORDER + 1], wxr[ORDER + 1];
+    if constexpr (sizeof(R) == 4) {
+      // the fractional offsets are exact in float64; the weights in float32
+      spline_weights_t<ORDER, R>((R)(cz - fz), wzr);
+      spline_weights_t<ORDER, R>((R)(cx - fx), wxr);
+    } else {
+      spline_weights_t<ORDER, double>(cz - fz, wzd);
+      spline_weights_t<ORDER, double>(cx - fx, wxd);
 #pragma unroll
-    for (int i = 0; i <= ORDER; ++i)
+      for (int i = 0; i <= ORDER; ++i) {
+        wzr[i] = (R)wzd[i];
+        wxr[i] = (R)wxd[i];
+      }
+    }
 #pragma unroll
-      for (int j = 0; j <= ORDER; ++j) w[i][j] = valid ? wz[i] * wx[j] : 0.0;
+    for (int i = 0; i <= ORDER; ++i) wzr[i] = valid ? wzr[i] : R(0);
     int zr[ORDER + 1], xc[ORDER + 1];
     rows_of<0>(sz, nz, wz0, zr);
     rows_of<0>(sx, nx, wx0, xc);
@@ -984,13 +997,13 @@ struct RotTile {
       gs[i] = swz(zr[i]) << 4;
       cb[i] = (unsigned)xc[i] * ELB;
     }
-    // tap sums in float32 with the float64 weights rounded once, each
-    // plane's sum widened into the float64 angle sum
+    // tap sums in the path's precision, each plane's sum widened into the
+    // float64 angle sum
     R wf[ORDER + 1][ORDER + 1];
 #pragma unroll
     for (int i = 0; i <= ORDER; ++i)
 #pragma unroll
-      for (int j = 0; j <= ORDER; ++j) wf[i][j] = (R)w[i][j];
+      for (int j = 0; j <= ORDER; ++j) wf[i][j] = wzr[i] * wxr[j];
 #pragma unroll
     for (int q = 0; q < YC; ++q) {
       V v = mul_w(wf[0][0], tap(tile, rb[0] + (cb[0] ^ gs[0]) + q * PLB));
