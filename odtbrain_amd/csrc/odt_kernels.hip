@@ -1124,6 +1124,11 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Whole-sample mirror of an index at most n - 1 outside [0, n).
+__device__ __forceinline__ int mirror1(int idx, int n) {
+  return idx < 0 ? -idx : (idx >= n ? 2 * n - 2 - idx : idx);
+}
+
 // ---------------------------------------------------------------------------
 // affine gather-accumulate for the tilted axis (_alg3d_bppt.py:904-944):
 // output voxel (z,y,x) is sample o = (x, Ny-1-y, z) of the (x, -y, z) frame;
@@ -1136,10 +1141,15 @@ __global__ void __launch_bounds__(256)
                  const double *__restrict__ params, V *__restrict__ out,
                  V *__restrict__ comp, int ny, int nz, int nx, int first,
                  int last) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  if (x >= nx || y >= ny) return;
+  // a wavefront takes an 8 x 8 (x, z) patch of one y plane: its taps then
+  // fall on a compact rotated patch of the slab (a few cache lines per
+  // gather) for every rotation angle, where a 64-voxel x row would spread
+  // over up to 64 lines near a quarter turn
+  const int lane = threadIdx.x & 63;
+  const int x = blockIdx.x * 8 + (lane & 7);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int z = blockIdx.z * 8 + (lane >> 3);
+  if (x >= nx || y >= ny || z >= nz) return;
   const size_t o = ((size_t)z * ny + y) * nx + x;
   wide_t<V> sum;
   if (!first)
@@ -1148,6 +1158,7 @@ __global__ void __launch_bounds__(256)
     acc_zero(sum);
   const double o0 = (double)x, o1 = (double)(ny - 1 - y), o2 = (double)z;
   const size_t vol = (size_t)ny * nz * nx;
+  const bool fast = nx >= 4 && ny >= 4 && nz >= 4;
   for (int a = 0; a < nb; ++a) {
     const double *p = params + 12 * a;
     const double c0 = affine3(o0, o1, o2, p[0], p[1], p[2], p[9]);
@@ -1164,11 +1175,15 @@ __global__ void __launch_bounds__(256)
     spline_weights<ORDER, R>(c2, w2);
     const int s0 = spline_start<ORDER>(c0), s1 = spline_start<ORDER>(c1),
               s2 = spline_start<ORDER>(c2);
+    // taps of an in-range coordinate leave the volume by at most
+    // (ORDER + 1) / 2 <= 3 samples: with every edge >= 4 one reflection is
+    // the whole mirror rule (no integer division)
 #pragma unroll
     for (int j = 0; j <= ORDER; ++j) {
-      i0[j] = mirror_idx(s0 + j, nx);
-      i1[j] = (size_t)(ny - 1 - mirror_idx(s1 + j, ny)) * nz * nx;
-      i2[j] = (size_t)mirror_idx(s2 + j, nz) * nx;
+      i0[j] = fast ? mirror1(s0 + j, nx) : mirror_idx(s0 + j, nx);
+      i1[j] = (size_t)(ny - 1 - (fast ? mirror1(s1 + j, ny) : mirror_idx(s1 + j, ny))) *
+              nz * nx;
+      i2[j] = (size_t)(fast ? mirror1(s2 + j, nz) : mirror_idx(s2 + j, nz)) * nx;
     }
     const V *base = coef + (size_t)a * vol;
     V v;
@@ -1584,8 +1599,8 @@ template <typename R, typename V, int ORDER>
 static hipError_t affine_dispatch(const V *coef, int nb, const double *params,
                                   V *out, V *comp, int ny, int nz, int nx,
                                   int first, int last, hipStream_t st) {
-  dim3 block(64, 4);
-  dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+  dim3 block(256);
+  dim3 grid((nx + 7) / 8, (ny + 3) / 4, (nz + 7) / 8);
   k_affine_acc<R, V, ORDER><<<grid, block, 0, st>>>(coef, nb, params, out, comp,
                                                     ny, nz, nx, first, last);
   return hipGetLastError();

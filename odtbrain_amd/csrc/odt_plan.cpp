@@ -314,11 +314,17 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
       }
       HIPCHK(hipEventRecord(p->ev_rot[2 * p->n_rot], st), "event");
     }
-    p->n_rot_k += tilted || p->order < 2 ? 1 : (nb + p->RG - 1) / p->RG;
+    p->n_rot_k += !tilted && p->order < 2 ? 1 : (nb + p->RG - 1) / p->RG;
     if (tilted)
-      HIPCHK((odt::launch_affine_acc<R, V>(p->order, slab, nb, prm, out, comp, g.Ny,
-                                           g.ln, g.Nx, first, last, st)),
-             "k_affine_acc");
+      for (int g0 = 0; g0 < nb; g0 += p->RG) {
+        const int gn = std::min(p->RG, nb - g0);
+        const size_t per_angle = (size_t)g.Ny * g.ln * g.Nx;
+        HIPCHK((odt::launch_affine_acc<R, V>(p->order, slab + (size_t)g0 * per_angle, gn,
+                                             prm + (size_t)g0 * p->pstride, out, comp,
+                                             g.Ny, g.ln, g.Nx, first && g0 == 0,
+                                             last && g0 + gn == nb, st)),
+               "k_affine_acc");
+      }
     else if (p->kind == ODT_KIND_3D && p->order < 2)
       HIPCHK((odt::launch_seq_rotate_sum<R, V>(p->order, slab, nb, prm, out, comp,
                                                g.Ny, g.ln, g.Nx, first, last, st)),
@@ -562,7 +568,7 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   // L2, so long batches are split (the running sums go through the output
   // and its compensation term between launches).
   p->RG = p->NB;
-  if (d->kind == ODT_KIND_3D && d->intp_order >= 2) {
+  if ((d->kind == ODT_KIND_3D || d->kind == ODT_KIND_3D_TILTED) && d->intp_order >= 2) {
     const char *env = std::getenv("ODTBRAIN_AMD_ROTATE_GROUP");
     const int rg = env ? std::atoi(env) : 48;
     if (rg > 0) p->RG = std::min(p->NB, rg);
