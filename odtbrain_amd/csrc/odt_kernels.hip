@@ -21,6 +21,8 @@
 // folded into the transforms), k_fft_batch backs the standalone transform
 // entry point.
 #include <algorithm>
+#include <cstdlib>
+#include <type_traits>
 
 #include "odt_device.h"
 #include "odt_fft8.h"
@@ -881,6 +883,10 @@ struct RotTile {
                                                int nyv, int nz, int nx, int wave,
                                                int lane, const unsigned *lo) {
     const char *src = reinterpret_cast<const char *>(coefA);
+    // the mask bounds the destination offsets for the compiler: it then
+    // knows these stores stay inside dst and does not wait for them before
+    // reads of another buffer
+    wave &= NW - 1;
     const int rlo = ext & 0xff, rhi = (ext >> 8) & 0xff;
     const int clo = (ext >> 16) & 0xff, chi = (ext >> 24) & 0xff;
     if constexpr (VEC == 4) {
@@ -950,6 +956,12 @@ struct RotTile {
     return t - w0;
   }
 
+  // Tap offsets (window bytes) and weights of one voxel for one angle.
+  struct Taps {
+    unsigned off[ORDER + 1][ORDER + 1];
+    R wf[ORDER + 1][ORDER + 1];
+  };
+
   // Gather one angle for the voxel (oz, ox) of every plane and add it to acc.
   // Branch-free: a voxel outside the rotated plane (scipy's constant mode,
   // value 0) or outside the volume gets zero weights and reads in-window
@@ -959,6 +971,16 @@ struct RotTile {
                                                  double oz, double ox, int nz,
                                                  int nx, bool own,
                                                  wide_t<V> *acc) {
+    Taps t;
+    prep(wz0, wx0, p, oz, ox, nz, nx, own, t);
+    gather(tile, t, acc);
+  }
+
+  // The tap set of the voxel (oz, ox) for one angle (coordinates, validity,
+  // weights and window offsets; the same for every plane of the window).
+  __device__ static __forceinline__ void prep(int wz0, int wx0, const double *p,
+                                              double oz, double ox, int nz,
+                                              int nx, bool own, Taps &t) {
     const double cz = affine2(oz, ox, p[0], p[1], p[4]);
     const double cx = affine2(oz, ox, p[2], p[3], p[5]);
     const bool valid = own && cz >= 0.0 && cz <= (double)(nz - 1) && cx >= 0.0 &&
@@ -997,22 +1019,27 @@ struct RotTile {
       gs[i] = swz(zr[i]) << 4;
       cb[i] = (unsigned)xc[i] * ELB;
     }
-    // tap sums in the path's precision, each plane's sum widened into the
-    // float64 angle sum
-    R wf[ORDER + 1][ORDER + 1];
 #pragma unroll
     for (int i = 0; i <= ORDER; ++i)
 #pragma unroll
-      for (int j = 0; j <= ORDER; ++j) wf[i][j] = wzr[i] * wxr[j];
+      for (int j = 0; j <= ORDER; ++j) {
+        t.wf[i][j] = wzr[i] * wxr[j];
+        t.off[i][j] = rb[i] + (cb[j] ^ gs[i]);
+      }
+  }
+
+  // Tap sums in the path's precision, each plane's sum widened into the
+  // float64 angle sum.
+  __device__ static __forceinline__ void gather(const char *tile, const Taps &t,
+                                                wide_t<V> *acc) {
 #pragma unroll
     for (int q = 0; q < YC; ++q) {
-      V v = mul_w(wf[0][0], tap(tile, rb[0] + (cb[0] ^ gs[0]) + q * PLB));
+      V v = mul_w(t.wf[0][0], tap(tile, t.off[0][0] + q * PLB));
 #pragma unroll
       for (int i = 0; i <= ORDER; ++i)
 #pragma unroll
         for (int j = 0; j <= ORDER; ++j)
-          if (i | j)
-            v = fma_w(wf[i][j], tap(tile, rb[i] + (cb[j] ^ gs[i]) + q * PLB), v);
+          if (i | j) v = fma_w(t.wf[i][j], tap(tile, t.off[i][j] + q * PLB), v);
       acc[q] = acc[q] + widen(v);
     }
   }
@@ -1114,6 +1141,103 @@ __global__ void __launch_bounds__(256)
                   sum);
       barrier_no_fence();
     }
+  }
+  if (own) {
+#pragma unroll
+    for (int q = 0; q < YC; ++q) {
+      if (q >= nyv) break;
+      store_sum(out, comp, ((size_t)z * ny + y0 + q) * nx + x, sum[q], !last);
+    }
+  }
+}
+
+// Four planes in two half buffers of two planes, pipelined at half-angle
+// granularity: planes 2-3 of angle a load while planes 0-1 of angle a are
+// gathered, planes 0-1 of angle a+1 while planes 2-3 of angle a are.  The
+// voxel's tap set (coordinates, weights, offsets) is formed once per angle
+// for all four planes.
+template <typename R, typename V, int ORDER, int VEC>
+__global__ void __launch_bounds__(256)
+    k_rotate_lds2(const V *__restrict__ coef, int nb,
+                  const double *__restrict__ params,
+                  const int4 *__restrict__ win, V *__restrict__ out,
+                  V *__restrict__ comp, int ny, int nz, int nx, int first,
+                  int last, int ntx, int ntz, int xcd_map) {
+  constexpr int YC = 4;
+  using RT = RotTile<R, V, ORDER, 2, VEC>;
+  // the halves as separate LDS objects: reads of one are not ordered behind
+  // the in-flight direct loads into the other
+  __shared__ __align__(16) char h0[RT::TB];
+  __shared__ __align__(16) char h1[RT::TB];
+  const int nt = ntx * ntz;
+  const int nyc = (ny + YC - 1) / YC;
+  int yc, ti;
+  if (xcd_map) {
+    const int bid = blockIdx.x;
+    yc = (bid & 7) + 8 * (bid / (8 * nt));
+    ti = (bid >> 3) % nt;
+    if (yc >= nyc) return;  // uniform per block
+  } else {
+    yc = blockIdx.x / nt;
+    ti = blockIdx.x - yc * nt;
+  }
+  const int tz = ti / ntx, tx = ti - tz * ntx;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int z = tz * kRotTile + (wave >> 1) * 8 + ((lane >> 5) << 2) + ((lane >> 2) & 3);
+  const int x = tx * kRotTile + (wave & 1) * 8 + (((lane >> 4) & 1) << 2) + (lane & 3);
+  const int y0 = yc * YC;
+  const int nyv = ny - y0 < YC ? ny - y0 : YC;
+  // planes of each half that exist; an empty upper half reads the lower
+  // half's first plane (in bounds, never stored)
+  const int nyv0 = nyv < 2 ? nyv : 2, nyv1 = nyv > 2 ? nyv - 2 : 0;
+  const bool own = x < nx && z < nz;
+  const size_t plane = (size_t)nz * nx;
+  unsigned lo[RT::NLO];
+  RT::lane_offsets(lo, wave, lane, nx);
+  wide_t<V> sum[YC];
+#pragma unroll
+  for (int q = 0; q < YC; ++q) {
+    acc_zero(sum[q]);
+    if (!first && own && q < nyv)
+      sum[q] = load_sum(out, comp, ((size_t)z * ny + y0 + q) * nx + x);
+  }
+  const double oz = (double)z, ox = (double)x;
+  const V *base0 = coef + (size_t)y0 * plane;
+  const V *base1 = nyv1 > 0 ? base0 + 2 * plane : base0;
+  const size_t aStride = (size_t)ny * plane;
+  const int4 *wt = win + ti;
+  int4 w = wt[0];  // nb >= 1 (the launcher checks)
+  RT::issue(base0, w.x, w.y, w.z, h0, plane, nyv0, nz, nx, wave, lane, lo);
+  // angles 0 .. nb-2 prefetch the next angle's lower half; the last angle
+  // (peeled) drains instead
+  for (int a = 0; a + 1 < nb; ++a) {
+    RT::issue(base1 + a * aStride, w.x, w.y, w.z, h1, plane, nyv1, nz, nx, wave,
+              lane, lo);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+    barrier_no_fence();
+    typename RT::Taps t;
+    RT::prep(w.x, w.y, params + 6 * a, oz, ox, nz, nx, own, t);
+    RT::gather(h0, t, sum);
+    barrier_no_fence();
+    w = wt[(size_t)(a + 1) * nt];
+    RT::issue(base0 + (a + 1) * aStride, w.x, w.y, w.z, h0, plane, nyv0, nz, nx,
+              wave, lane, lo);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+    barrier_no_fence();
+    RT::gather(h1, t, sum + 2);
+    barrier_no_fence();
+  }
+  {
+    const int a = nb - 1;
+    RT::issue(base1 + a * aStride, w.x, w.y, w.z, h1, plane, nyv1, nz, nx, wave,
+              lane, lo);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    barrier_no_fence();
+    typename RT::Taps t;
+    RT::prep(w.x, w.y, params + 6 * a, oz, ox, nz, nx, own, t);
+    RT::gather(h0, t, sum);
+    RT::gather(h1, t, sum + 2);
   }
   if (own) {
 #pragma unroll
@@ -1533,34 +1657,57 @@ constexpr int rot_yc_max() {
   return 2 * 4 * kRotWin * kRotWin * (int)sizeof(V) <= 32768 ? 4 : 2;
 }
 
-template <typename R, typename V, int ORDER, int YC>
+// Complex samples with at least four planes take k_rotate_lds2 (four
+// planes, pipelined half buffers); measured at 256^2 x 360 (complex64,
+// 48 angles per launch) 2.02 ms per launch against 2.10 ms for
+// k_rotate_lds with two planes.  ODTBRAIN_AMD_ROTATE_TWO_PLANE forces
+// k_rotate_lds (for comparisons).
+inline bool rotate_two_plane() {
+  static const bool f = std::getenv("ODTBRAIN_AMD_ROTATE_TWO_PLANE") != nullptr;
+  return f;
+}
+
+template <typename R, typename V, int ORDER, int YC, bool PIPE = false>
 static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
                                     int *win, V *out, V *comp, int ny, int nz,
                                     int nx, int first, int last, hipStream_t st) {
+  static_assert(!PIPE || YC == 4, "the pipelined kernel holds four planes");
+  if (nb < 1) return hipSuccess;
   const int ntx = (nx + kRotTile - 1) / kRotTile;
   const int ntz = (nz + kRotTile - 1) / kRotTile;
   const int nt = ntx * ntz;
-  // 16-byte loads need windows inside the plane and 16-byte aligned rows
-  constexpr int AL = 16 / (int)sizeof(V);
-  const bool vec4 = nz >= kRotWin && nx >= kRotWin && nx % AL == 0 &&
-                    ((uintptr_t)coef & 15) == 0;
   const long long nw = (long long)nb * nt;
-  k_rot_windows<ORDER><<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(
-      params, nb, ntz, ntx, nz, nx, vec4 ? AL : 1, (int4 *)win);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   const int nyc = (ny + YC - 1) / YC;
   const int xcd_map = nyc >= 8 ? 1 : 0;
   const long long blocks =
       xcd_map ? 8LL * nt * ((nyc + 7) / 8) : (long long)nt * nyc;
-  if (vec4)
-    k_rotate_lds<R, V, ORDER, YC, 4><<<(unsigned)blocks, 256, 0, st>>>(
-        coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
-        ntx, ntz, xcd_map);
-  else
-    k_rotate_lds<R, V, ORDER, YC, 1><<<(unsigned)blocks, 256, 0, st>>>(
-        coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
-        ntx, ntz, xcd_map);
+  // 16-byte loads need windows inside the plane and 16-byte aligned rows
+  constexpr int AL = 16 / (int)sizeof(V);
+  const bool vec4 = nz >= kRotWin && nx >= kRotWin && nx % AL == 0 &&
+                    ((uintptr_t)coef & 15) == 0;
+  k_rot_windows<ORDER><<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(
+      params, nb, ntz, ntx, nz, nx, vec4 ? AL : 1, (int4 *)win);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if constexpr (PIPE) {
+    if (vec4)
+      k_rotate_lds2<R, V, ORDER, 4><<<(unsigned)blocks, 256, 0, st>>>(
+          coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
+          ntx, ntz, xcd_map);
+    else
+      k_rotate_lds2<R, V, ORDER, 1><<<(unsigned)blocks, 256, 0, st>>>(
+          coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
+          ntx, ntz, xcd_map);
+  } else {
+    if (vec4)
+      k_rotate_lds<R, V, ORDER, YC, 4><<<(unsigned)blocks, 256, 0, st>>>(
+          coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
+          ntx, ntz, xcd_map);
+    else
+      k_rotate_lds<R, V, ORDER, YC, 1><<<(unsigned)blocks, 256, 0, st>>>(
+          coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
+          ntx, ntz, xcd_map);
+  }
   return hipGetLastError();
 }
 
@@ -1568,6 +1715,12 @@ template <typename R, typename V, int ORDER>
 static hipError_t rotate_dispatch_yc(const V *coef, int nb, const double *params,
                                      int *win, V *out, V *comp, int ny, int nz,
                                      int nx, int first, int last, hipStream_t st) {
+  if constexpr (std::is_same<V, cpx<R>>::value) {
+    if (ny >= 4 && !rotate_two_plane())
+      return rotate_lds_launch<R, V, ORDER, 4, true>(coef, nb, params, win, out,
+                                                     comp, ny, nz, nx, first,
+                                                     last, st);
+  }
   constexpr int YM = rot_yc_max<V>();
   if (ny >= YM)
     return rotate_lds_launch<R, V, ORDER, YM>(coef, nb, params, win, out, comp,
