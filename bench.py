@@ -506,7 +506,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(),
-                     "kernel": "k_rotate_lds", "ms_per_launch": rot_ms,
+                     "kernel": "k_rotate_lds2", "ms_per_launch": rot_ms,
                      "launches_per_step": launches,
                      "angles_per_launch": ang_per_launch,
                      "algorithmic_bytes_per_launch": alg_bytes},
