@@ -13,7 +13,7 @@
 //                the fused z-prefilter), inverse transform along y, crop (T)
 //   k_row_ifft   inverse transform along x, crop, fused x-prefilter; writes
 //                the B-spline coefficient slab of the projection
-//   k_rotate_lds / k_affine_acc
+//   k_rotate_lds2 / k_rotate_lds / k_affine_acc
 //                resample every slab of the batch at the rotated position
 //                of each output voxel and sum over angles (compensated)
 //
