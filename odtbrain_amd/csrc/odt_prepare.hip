@@ -14,7 +14,8 @@
 //   segmented radix sort (rocPRIM, stable: equal keys keep creation order)
 //   k_uw_gather        one thread per image merges groups edge by edge in
 //                      sorted order (union by size with whole-turn offsets,
-//                      the published merge rules)
+//                      the published merge rules); a helper wavefront
+//                      warms L2 with the nodes of the edges just ahead
 //   k_uw_apply         value + 2 pi * increment
 //   k_min_*, k_align   align_unwrapped: steps of the unwrapped corner
 //                      samples along the angles, their mode per angle,
@@ -197,21 +198,66 @@ __device__ __forceinline__ int uw_find(int p, int2 *node, int *inc, int2 *root) 
   return r;
 }
 
-// One thread per image: the groups of the published algorithm as a
-// union-find forest (union by size).  The sorted edges carry their
-// whole-turn step and are read 16 at a time ahead of use.
+// One image per block: one thread merges the groups of the published
+// algorithm as a union-find forest (union by size), edge by edge in sorted
+// order; the sorted edges carry their whole-turn step and are read 16 at a
+// time ahead of use.  A second wavefront walks at most kUwAhead edges
+// ahead of the merging thread and touches the nodes of those edges with
+// agent-scope loads: they fill L2 (the node arrays of all images exceed
+// it) but not the merging wavefront's L1, so no stale line can reach the
+// merges; the merging thread never waits for it.  The helper's wait ends
+// because the merging thread always runs to the last edge and then
+// publishes that.
+constexpr int kUwAhead = 32;
+constexpr int kUwThreads = 128;
+
 template <typename R>
-__global__ void k_uw_gather(int A, int H, int W, const int *__restrict__ sorted,
-                            int2 *node_all) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= A) return;
+__global__ void __launch_bounds__(kUwThreads)
+    k_uw_gather(int A, int H, int W, const int *__restrict__ sorted,
+                int2 *node_all) {
+  const int a = blockIdx.x;
+  if (a >= A) return;  // uniform per block
   const long long npix = (long long)H * W;
   const long long E = (long long)H * (W - 1) + (long long)(H - 1) * W;
   int2 *node = node_all + a * npix;
-  for (int p = 0; p < npix; ++p) node[p] = make_int2(-1, 0);
+  for (long long p = threadIdx.x; p < npix; p += kUwThreads)
+    node[p] = make_int2(-1, 0);
   const int *es = sorted + a * E;
+  __shared__ long long progress;  // edges the merging thread has finished
+  if (threadIdx.x == 0) progress = 0;
+  __syncthreads();
+  if (threadIdx.x == 64) {
+    // batches of HB edges: all 2 HB loads are in flight before any result
+    // is used, so the helper outruns the merges it serves
+    constexpr int HB = 8;
+    int touch = 0;
+    for (long long k0 = 0; k0 < E; k0 += HB) {
+      while (k0 > __hip_atomic_load(&progress, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP) +
+                       kUwAhead)
+        __builtin_amdgcn_s_sleep(2);
+      int t[2 * HB];
+#pragma unroll
+      for (int u = 0; u < HB; ++u) {
+        const long long k = k0 + u < E ? k0 + u : E - 1;
+        int p1, p2;
+        uw_edge(es[k] >> 2, H, W, &p1, &p2);
+        t[2 * u] = __hip_atomic_load(&node[p1].x, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        t[2 * u + 1] = __hip_atomic_load(&node[p2].x, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+      }
+#pragma unroll
+      for (int u = 0; u < 2 * HB; ++u) touch += t[u];
+    }
+    __asm__ volatile("" : : "v"(touch));  // keeps the touching loads
+    return;
+  }
+  if (threadIdx.x != 0) return;
   constexpr int CH = 16;
   for (long long k0 = 0; k0 < E; k0 += CH) {
+    __hip_atomic_store(&progress, k0, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_WORKGROUP);
     int ev[CH];
 #pragma unroll
     for (int u = 0; u < CH; ++u) ev[u] = k0 + u < E ? es[k0 + u] : -1;
@@ -242,6 +288,8 @@ __global__ void k_uw_gather(int A, int H, int W, const int *__restrict__ sorted,
       }
     }
   }
+  __hip_atomic_store(&progress, E, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // value + 2 pi * increment, in float64 as the published code, stored as R
@@ -485,7 +533,7 @@ hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const vo
     int2 *node = (int2 *)(w + L.parent);
     // one image per wavefront: images advance independently (no lanes
     // waiting on each other's longer find paths)
-    k_uw_gather<R><<<A, 1, 0, st>>>(A, H, W, v1, node);
+    k_uw_gather<R><<<A, kUwThreads, 0, st>>>(A, H, W, v1, node);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_uw_apply<R><<<blocks_of(n), 256, 0, st>>>(phase, A, H, W, node);
     if ((e = hipGetLastError()) != hipSuccess) return e;
