@@ -86,6 +86,22 @@ def test_3d_orders_and_padding_vs_oracle(order, padval):
     assert rel_l2(got32, ref) < FP32_REL_L2
 
 
+@pytest.mark.parametrize("nx,ny", [(32, 15), (34, 7), (36, 4), (32, 3)])
+def test_3d_rotate_plane_chunks_vs_oracle(nx, ny):
+    """y sizes around the four-plane chunks of the complex rotate kernel:
+    a last chunk of 3 planes (its upper half holds one), exactly one chunk,
+    and fewer than four planes (the two-plane kernel); x and z >= 32 take
+    the 16-byte window copies."""
+    sino, angles = create_test_sino_3d(A=9, Nx=nx, Ny=ny)
+    kw = dict(res=3.0, nm=1.335, lD=0.5)
+    ref = O.backpropagate_3d(sino, angles, **kw)
+    got = G.backpropagate_3d(sino, angles, **kw)
+    assert got.shape == ref.shape == (nx, ny, nx)
+    assert rel_l2(got, ref) < FP64_REL_L2
+    got32 = G.backpropagate_3d(sino, angles, dtype=np.float32, **kw)
+    assert rel_l2(got32, ref) < FP32_REL_L2
+
+
 def test_3d_iir_prefilter_equals_fused():
     sino, angles = create_test_sino_3d(A=11, Nx=20, Ny=16)
     for order in (2, 3):
