@@ -325,6 +325,63 @@ def tilted_bench(steps, warmup):
     print(json.dumps(line), flush=True)
 
 
+def example_bench(path, steps, warmup):
+    """The reference's 3D example pipeline on one of its example archives
+    (a local ``.tar.lzma`` file, e.g. fdtd_3d_sino_A180_R6.500.tar.lzma):
+    sinogram_as_rytov -> backpropagate_3d (res, nm, lD from the archive) ->
+    odt_to_ri, inputs resident on the GPU; FP32 and FP64, the FP32
+    refractive index against FP64 and both against the archive's phantom."""
+    import torch
+    import odtbrain_amd as G
+    from odtbrain_amd import example_data
+    dev = torch.device("cuda:0")
+    log(f"bench: loading {path}")
+    sino, angles, phantom, cfg = example_data.load_data(path)
+    res, nm, lD = cfg["res"], cfg["nm"], cfg.get("lD", 0.0)
+    d64 = torch.from_numpy(np.ascontiguousarray(sino)).to(dev)
+    d32 = d64.to(torch.complex64)
+
+    def step(d, dtype):
+        uR = G.sinogram_as_rytov(d)
+        f = G.backpropagate_3d(uR, angles, res, nm, lD=lD, dtype=dtype)
+        return G.odt_to_ri(f, res, nm)
+    for _ in range(warmup):
+        step(d32, np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ri32 = step(d32, np.float32)
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / steps
+    ri64 = step(d64, np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ri64 = step(d64, np.float64)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    err = float(torch.linalg.vector_norm(ri32.to(torch.complex128) - ri64) /
+                torch.linalg.vector_norm(ri64))
+    ph = torch.from_numpy(phantom).to(dev)
+    A, N = sino.shape[0], sino.shape[-1]
+    line = {
+        "metric": "example pipeline (sinogram_as_rytov + backpropagate_3d + "
+                  "odt_to_ri) time", "value": t32 * 1e3, "unit": "ms",
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": t32 * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"example archive {pathlib.Path(path).name} ({A} x {N}^2)",
+        "config": {"workload": "reference example pipeline",
+                   "res": res, "nm": nm, "lD": lD},
+        "voxel_angle_per_s": N ** 3 * A / t32,
+        "fp64_gpu": {"ms_per_step": t64 * 1e3,
+                     "fp32_ri_rel_l2_vs_fp64": err},
+        "ri_vs_phantom_mean_abs": {
+            "fp32": float((ri32.real.double() - ph).abs().mean()),
+            "fp64": float((ri64.real - ph).abs().mean())},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,6 +400,9 @@ def main():
     ap.add_argument("--apple", action="store_true",
                     help="run apple.correct on the 256^3 reconstruction "
                          "instead (--steps iterations, default 100)")
+    ap.add_argument("--example", metavar="PATH",
+                    help="run the reference's 3D example pipeline on a local "
+                         "example archive (.tar.lzma) instead")
     ap.add_argument("--fd-directions", type=int, default=64,
                     help="finite-difference directions of --adjoint")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -355,6 +415,9 @@ def main():
                           "and is not benchmarked"}))
         sys.exit(2)
 
+    if args.example:
+        example_bench(args.example, args.steps, args.warmup)
+        return
     if args.batch:
         batch_bench(args.steps, args.warmup)
         return
