@@ -1063,14 +1063,29 @@ __device__ __forceinline__ void barrier_no_fence() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// ag > 0: angle groups along blockIdx.y; group gy takes angles
+// [gy*ag, gy*ag + ag) and writes its own running sum and compensation at
+// out/comp + gy*gstride (k_fold_groups adds the groups up).
 template <typename R, typename V, int ORDER, int YC, int VEC>
 __global__ void __launch_bounds__(256)
     k_rotate_lds(const V *__restrict__ coef, int nb,
                  const double *__restrict__ params,
                  const int4 *__restrict__ win, V *__restrict__ out,
                  V *__restrict__ comp, int ny, int nz, int nx, int first,
-                 int last, int ntx, int ntz, int xcd_map) {
+                 int last, int ntx, int ntz, int xcd_map, int ag,
+                 size_t gstride) {
   using RT = RotTile<R, V, ORDER, YC, VEC>;
+  if (ag > 0) {
+    const int a0 = blockIdx.y * ag;
+    nb = min(ag, nb - a0);
+    coef += (size_t)a0 * ny * nz * nx;
+    params += 6 * (size_t)a0;
+    win += (size_t)a0 * ntx * ntz;
+    out += blockIdx.y * gstride;
+    comp += blockIdx.y * gstride;
+    first = 1;
+    last = 0;
+  }
   // two tile buffers as separate LDS objects: reads of one are not ordered
   // behind the in-flight direct loads into the other
   __shared__ __align__(16) char bufA[RT::TB];
@@ -1149,6 +1164,24 @@ __global__ void __launch_bounds__(256)
       store_sum(out, comp, ((size_t)z * ny + y0 + q) * nx + x, sum[q], !last);
     }
   }
+}
+
+// Adds the G angle-group sums of k_rotate_lds (sum g at part + g*n, its
+// compensation at part + (G + g)*n) to the running sum in out/comp, in group
+// order, in float64.
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_fold_groups(V *__restrict__ out, V *__restrict__ comp,
+                  const V *__restrict__ part, int G, size_t n, int first,
+                  int last) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  wide_t<V> s;
+  acc_zero(s);
+  if (!first) s = load_sum(out, comp, i);
+  for (int g = 0; g < G; ++g)
+    s = s + (widen(part[(size_t)g * n + i]) - widen(part[(size_t)(G + g) * n + i]));
+  store_sum(out, comp, i, s, !last);
 }
 
 // Four planes in two half buffers of two planes, pipelined at half-angle
@@ -1667,10 +1700,16 @@ inline bool rotate_two_plane() {
   return f;
 }
 
+// groups > 1 (with part: room for 2 * groups volumes): the angles are split
+// into that many contiguous groups that run side by side (grid y), each
+// into its own compensated sum in part, and k_fold_groups adds them to out
+// in group order.  For planes with few tiles (2D images), whose blocks
+// would otherwise each loop over every angle on a fraction of the CUs.
 template <typename R, typename V, int ORDER, int YC, bool PIPE = false>
 static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
                                     int *win, V *out, V *comp, int ny, int nz,
-                                    int nx, int first, int last, hipStream_t st) {
+                                    int nx, int first, int last, hipStream_t st,
+                                    int groups = 1, V *part = nullptr) {
   static_assert(!PIPE || YC == 4, "the pipelined kernel holds four planes");
   if (nb < 1) return hipSuccess;
   const int ntx = (nx + kRotTile - 1) / kRotTile;
@@ -1698,15 +1737,32 @@ static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
       k_rotate_lds2<R, V, ORDER, 1><<<(unsigned)blocks, 256, 0, st>>>(
           coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
           ntx, ntz, xcd_map);
+  } else if (groups > 1 && part && nb > 1) {
+    const int ag = (nb + groups - 1) / groups;
+    const int G = (nb + ag - 1) / ag;  // every group non-empty
+    const size_t n = (size_t)ny * nz * nx;
+    const dim3 grid((unsigned)blocks, (unsigned)G);
+    if (vec4)
+      k_rotate_lds<R, V, ORDER, YC, 4><<<grid, 256, 0, st>>>(
+          coef, nb, params, (const int4 *)win, part, part + (size_t)G * n, ny, nz,
+          nx, 1, 0, ntx, ntz, xcd_map, ag, n);
+    else
+      k_rotate_lds<R, V, ORDER, YC, 1><<<grid, 256, 0, st>>>(
+          coef, nb, params, (const int4 *)win, part, part + (size_t)G * n, ny, nz,
+          nx, 1, 0, ntx, ntz, xcd_map, ag, n);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_fold_groups<V><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
+        out, comp, part, G, n, first, last);
   } else {
     if (vec4)
       k_rotate_lds<R, V, ORDER, YC, 4><<<(unsigned)blocks, 256, 0, st>>>(
           coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
-          ntx, ntz, xcd_map);
+          ntx, ntz, xcd_map, 0, 0);
     else
       k_rotate_lds<R, V, ORDER, YC, 1><<<(unsigned)blocks, 256, 0, st>>>(
           coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last,
-          ntx, ntz, xcd_map);
+          ntx, ntz, xcd_map, 0, 0);
   }
   return hipGetLastError();
 }
@@ -1714,7 +1770,8 @@ static hipError_t rotate_lds_launch(const V *coef, int nb, const double *params,
 template <typename R, typename V, int ORDER>
 static hipError_t rotate_dispatch_yc(const V *coef, int nb, const double *params,
                                      int *win, V *out, V *comp, int ny, int nz,
-                                     int nx, int first, int last, hipStream_t st) {
+                                     int nx, int first, int last, hipStream_t st,
+                                     int groups, V *part) {
   if constexpr (std::is_same<V, cpx<R>>::value) {
     if (ny >= 4 && !rotate_two_plane())
       return rotate_lds_launch<R, V, ORDER, 4, true>(coef, nb, params, win, out,
@@ -1729,21 +1786,21 @@ static hipError_t rotate_dispatch_yc(const V *coef, int nb, const double *params
     return rotate_lds_launch<R, V, ORDER, 2>(coef, nb, params, win, out, comp,
                                              ny, nz, nx, first, last, st);
   return rotate_lds_launch<R, V, ORDER, 1>(coef, nb, params, win, out, comp, ny,
-                                           nz, nx, first, last, st);
+                                           nz, nx, first, last, st, groups, part);
 }
 
 template <typename R, typename V>
 hipError_t launch_rotate_acc(int order, const V *coef, int nb,
                              const double *params, int *win, V *out, V *comp,
                              int ny, int nz, int nx, int first, int last,
-                             hipStream_t st) {
+                             hipStream_t st, int groups, V *part) {
   switch (order) {
-    case 0: return rotate_dispatch_yc<R, V, 0>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
-    case 1: return rotate_dispatch_yc<R, V, 1>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
-    case 2: return rotate_dispatch_yc<R, V, 2>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
-    case 3: return rotate_dispatch_yc<R, V, 3>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
-    case 4: return rotate_dispatch_yc<R, V, 4>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
-    case 5: return rotate_dispatch_yc<R, V, 5>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 0: return rotate_dispatch_yc<R, V, 0>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st, groups, part);
+    case 1: return rotate_dispatch_yc<R, V, 1>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st, groups, part);
+    case 2: return rotate_dispatch_yc<R, V, 2>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st, groups, part);
+    case 3: return rotate_dispatch_yc<R, V, 3>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st, groups, part);
+    case 4: return rotate_dispatch_yc<R, V, 4>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st, groups, part);
+    case 5: return rotate_dispatch_yc<R, V, 5>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st, groups, part);
   }
   return hipErrorInvalidValue;
 }
@@ -1913,10 +1970,10 @@ hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch,
   template hipError_t launch_rotate_acc<R, R>(int, const R *, int,             \
                                               const double *, int *, R *, R *, \
                                               int, int, int, int, int,         \
-                                              hipStream_t);                    \
+                                              hipStream_t, int, R *);          \
   template hipError_t launch_rotate_acc<R, cpx<R>>(                            \
       int, const cpx<R> *, int, const double *, int *, cpx<R> *, cpx<R> *,     \
-      int, int, int, int, int, hipStream_t);                                   \
+      int, int, int, int, int, hipStream_t, int, cpx<R> *);                    \
   template hipError_t launch_affine_acc<R, R>(int, const R *, int,             \
                                               const double *, R *, R *, int,   \
                                               int, int, int, int,              \

@@ -55,12 +55,15 @@ hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
                            int slab0, int ldt, hipStream_t st);
 // Rotate-accumulate of nb angles; win is scratch of
 // 4 * nb * rotate_window_entries(nz, nx) ints (window origin and extents per tile).
+// groups > 1 (single-plane volumes): the angles run in that many concurrent
+// groups, each summed into part (room for 2 * groups volumes of ny*nz*nx),
+// then added to out in group order.
 size_t rotate_window_entries(int nz, int nx);
 template <typename R, typename V>
 hipError_t launch_rotate_acc(int order, const V *coef, int nb,
                              const double *params, int *win, V *out, V *comp,
                              int ny, int nz, int nx, int first, int last,
-                             hipStream_t st);
+                             hipStream_t st, int groups = 1, V *part = nullptr);
 template <typename R, typename V>
 hipError_t launch_affine_acc(int order, const V *coef, int nb,
                              const double *params, V *out, V *comp, int ny,

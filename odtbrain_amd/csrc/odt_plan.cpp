@@ -124,7 +124,8 @@ struct odt_plan {
   int ldt;                    // row stride of T (support columns, padded)
   int nb_fr, nb_fc, nb_col, nb_row;  // transforms per block
   size_t esz_c, esz_v;        // bytes of complex element / slab element
-  size_t off_F1, off_S, off_T, off_slab, off_comp, off_win, ws_bytes;
+  size_t off_F1, off_S, off_T, off_slab, off_comp, off_win, off_part, ws_bytes;
+  int RGG = 1;  // concurrent angle groups of the 2D rotate-accumulate
   // device tables
   void *twx = nullptr, *twy = nullptr, *P = nullptr, *aux = nullptr;
   void *q0x = nullptr, *q0y = nullptr, *q0z = nullptr;
@@ -337,7 +338,8 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
                    p->order, slab + (size_t)g0 * per_angle, gn,
                    prm + (size_t)g0 * p->pstride, (int *)(ws + p->off_win), out,
                    comp, g.Ny, g.ln, g.Nx, first && g0 == 0,
-                   last && g0 + gn == nb, st)),
+                   last && g0 + gn == nb, st, p->RGG,
+                   p->off_part ? (V *)(ws + p->off_part) : nullptr)),
                "k_rotate_acc");
       }
     if (p->timing) HIPCHK(hipEventRecord(p->ev_rot[2 * p->n_rot + 1], st), "event");
@@ -585,6 +587,24 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   p->off_win = off;
   off = align256(off + 4 * sizeof(int) * (size_t)p->NB *
                            odt::rotate_window_entries(p->g.ln, p->g.Nx));
+  // 2D images have few tiles ((N/16)^2: 64 at N = 128) and each block loops
+  // over every angle, so the angles run in concurrent groups (at least 8
+  // angles each, about 2560 blocks in all, a sum and a compensation volume
+  // per group) that are added up in group order afterwards.
+  p->RGG = 1;
+  p->off_part = 0;
+  if (d->kind == ODT_KIND_2D) {
+    const size_t tiles = odt::rotate_window_entries(p->g.ln, p->g.Nx);
+    int gg = (int)((2560 + tiles - 1) / tiles);
+    gg = std::min(gg, std::max(1, p->NB / 8));
+    if (const char *genv = std::getenv("ODTBRAIN_AMD_ROTATE_2D_GROUPS"))
+      gg = std::max(1, std::atoi(genv));
+    if (gg > 1) {
+      p->RGG = gg;
+      p->off_part = off;
+      off = align256(off + 2 * (size_t)gg * slab_angle);
+    }
+  }
   p->ws_bytes = off;
 
   int rc;
