@@ -230,6 +230,22 @@ def test_tilted_points_input_and_iir():
     assert rel_l2(got2, ref) < FP64_REL_L2
 
 
+@pytest.mark.parametrize("A,N", [(43, 64), (16, 40)])
+def test_2d_angle_groups_vs_oracle(A, N):
+    """2D plans with enough angles run the rotate-accumulate in concurrent
+    angle groups (43 angles on a 64^2 image: 5 groups of 9, the last of 7;
+    16 angles: 2 groups of 8) folded in group order."""
+    sino, angles = create_test_sino_2d(A=A, N=N, ampl_range=(0.9, 1.1))
+    kw = dict(res=3.3, nm=1.34, lD=1.0)
+    ref = O.backpropagate_2d(sino, angles, **kw)
+    got = G.backpropagate_2d(sino, angles, **kw)
+    assert rel_l2(got, ref) < FP64_REL_L2
+    got32 = G.backpropagate_2d(sino, angles, dtype=np.float32, **kw)
+    assert rel_l2(got32, ref) < FP32_REL_L2
+    r = G.backpropagate_2d(sino, angles, onlyreal=True, **kw)
+    assert np.allclose(r, got.real, rtol=0, atol=1e-12 * np.abs(got).max())
+
+
 @pytest.mark.parametrize("padval", [0, None, 1.5])
 @pytest.mark.parametrize("padding", [True, False])
 def test_2d_vs_oracle(padval, padding):
