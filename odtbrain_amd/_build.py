@@ -7,6 +7,7 @@ import pathlib
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
@@ -31,21 +32,30 @@ def hipcc():
     return exe
 
 
-def build(verbose=False):
-    """Compile every source to an object and link the shared library."""
+def build(verbose=False, jobs=None):
+    """Compile every source to an object (in parallel) and link the shared
+    library."""
     objdir = HERE / "_build"
     objdir.mkdir(exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         obj = objdir / (src + ".o")
         cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(INCLUDE), "-I", str(CSRC),
                "-c", str(CSRC / src), "-o", str(obj)]
         if src.endswith(".cpp"):
             cmd[1:1] = ["-x", "hip"]
+        cmds.append(cmd)
+        objs.append(str(obj))
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1, 8)
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs,
            "-o", str(tmp)]
