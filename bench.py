@@ -4,9 +4,15 @@
 A step is one full backpropagate_3d call (pad, FFT, filter, per-z
 propagation and inverse transform, fused prefilter, rotate-accumulate over
 all 360 angles) on a complex64 sinogram already resident in HBM, computed
-in float32.  Prints one JSON line.
+in float32.  Prints one JSON line with, beside the FP32 throughput:
+the FP64 GPU path; the rotate-accumulate kernel's HBM roofline; the CPU
+baseline (the oracle on all host cores, in full, with a bounded-sample
+extrapolation beside it) and the FP32/FP64 error against its complex128
+volume; the 64^2 x 100 workload (GPU and CPU); the prologue and epilogue
+of the reference's pipeline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--cpu-sample-only] [--no-fp64] [--no-c1]
 """
 import argparse
 import json
@@ -31,38 +37,139 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def kernel_source_hash():
+    """sha256 (first 16 hex digits) of the HIP sources the rotate-accumulate
+    kernel is built from; the committed traffic summary carries the hash of
+    the sources it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("odt_kernels.hip", "odt_device.h", "odt_launch.h"):
+        h.update((ROOT / "odtbrain_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
 def traffic_from_profile():
     """HBM bytes per rotate-accumulate launch measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), read
-    from the committed summary if present."""
+    from the committed summary; None when the summary is missing or was
+    measured on other kernel sources than the ones built here."""
     f = ROOT / "profiles" / "rotate_acc_c2_traffic.txt"
     if not f.exists():
-        return None
+        return None, "no traffic summary"
+    vals = {}
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+        if ":" in line:
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    src = vals.get("kernel_sources_sha256_16")
+    if src != kernel_source_hash():
+        return None, "traffic summary measured on other kernel sources"
+    return float(vals["traffic_bytes_per_launch"]), f"profiles/{f.name}"
 
 
-def cpu_baseline(uB, angles, seconds):
+def cpu_cores():
+    """Host cores available to this process (the CPU share of the GPU host;
+    os.cpu_count() and the affinity mask agree there)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(uB, angles, sample_angles, full):
     """Oracle (CPU restatement of the reference, float64, y-chunk process
-    pool + threaded FFTs) on a bounded, contiguous sample of the angles;
-    per-angle cost extrapolated linearly to all angles plus the setup."""
+    pool over all host cores + FFTs threaded over the same cores), timed per
+    call including pool and filter set-up as the reference runs: first a
+    bounded contiguous sample of ``sample_angles`` angles extrapolated
+    linearly, then (``full``) the whole reconstruction, whose output also
+    serves as the accuracy reference.  Returns (baseline dict, volume or
+    None)."""
     from oracle import odt_oracle
-    cores = min(16, os.cpu_count() or 1)
-    stats = {}
-    odt_oracle.backpropagate_3d(uB, angles, RES, NM, padval="edge",
-                                intp_order=2, num_cores=cores,
-                                time_limit=seconds, stats=stats)
-    per_angle = stats["loop_s"] / stats["angles"]
-    total = stats["setup_s"] + per_angle * len(angles)
+    cores = cpu_cores()
+    kw = dict(padval="edge", intp_order=2, num_cores=cores)
+    uB = uB.astype(np.complex128)
     V = N_DET ** 3
-    return {"value": V * len(angles) / total, "unit": "voxel-angle updates/s",
-            "cores": cores, "kind": "port",
-            "sample": (f"oracle float64, {stats['angles']} of {len(angles)} "
-                       f"angles ({stats['loop_s']:.1f} s) + setup "
-                       f"{stats['setup_s']:.1f} s, extrapolated: "
-                       f"{total:.1f} s per reconstruction")}
+    stats = {}
+    odt_oracle.backpropagate_3d(uB, angles, RES, NM,
+                                angle_range=(0, sample_angles), stats=stats,
+                                **kw)
+    per_angle = stats["loop_s"] / stats["angles"]
+    t_sample = stats["setup_s"] + per_angle * len(angles)
+    sample = {"value": V * len(angles) / t_sample,
+              "angles": stats["angles"], "loop_s": stats["loop_s"],
+              "setup_s": stats["setup_s"],
+              "extrapolated_s": t_sample}
+    ref = None
+    if full:
+        t0 = time.perf_counter()
+        ref = odt_oracle.backpropagate_3d(uB, angles, RES, NM, **kw)
+        t_full = time.perf_counter() - t0
+        out = {"value": V * len(angles) / t_full,
+               "unit": "voxel-angle updates/s", "cores": cores,
+               "kind": "port",
+               "sample": (f"oracle float64 (reference structure: Pool({cores}) "
+                          f"y-chunk rotation, FFTs on {cores} threads), the "
+                          f"whole 256^2 x 360 reconstruction: {t_full:.1f} s "
+                          f"per call; bounded sample of {stats['angles']} "
+                          f"angles extrapolated: {t_sample:.1f} s"),
+               "full_s": t_full, "bounded_sample": sample}
+    else:
+        out = {"value": sample["value"], "unit": "voxel-angle updates/s",
+               "cores": cores, "kind": "port",
+               "sample": (f"oracle float64, {stats['angles']} of "
+                          f"{len(angles)} angles ({stats['loop_s']:.1f} s) + "
+                          f"setup {stats['setup_s']:.1f} s, extrapolated: "
+                          f"{t_sample:.1f} s per reconstruction")}
+    return out, ref
+
+
+def c1_workload(cpu):
+    """The CPU-runnable workload (64^2 detector, 100 angles, centred sphere
+    r=16 px n=1.343, Rytov data with noise 0.01, edge padding to 128^2,
+    order 2): GPU FP32/FP64 time per call and error against the oracle,
+    the oracle run in full on the host cores beside it."""
+    import torch
+    import odtbrain_amd
+    from odtbrain_amd import phantoms
+    from oracle import odt_oracle
+    uB, angles = phantoms.sphere_sinogram_3d()
+    kw = dict(padval="edge", intp_order=2)
+    d64 = torch.from_numpy(uB).cuda()
+    d32 = d64.to(torch.complex64)
+    res = {}
+    for name, d, dt in (("fp32", d32, np.float32), ("fp64", d64, np.float64)):
+        for _ in range(2):
+            out = odtbrain_amd.backpropagate_3d(d, angles, RES, NM, dtype=dt,
+                                                **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            out = odtbrain_amd.backpropagate_3d(d, angles, RES, NM, dtype=dt,
+                                                **kw)
+        torch.cuda.synchronize()
+        res[name] = ((time.perf_counter() - t0) / 10, out.cpu().numpy())
+    V = 64 ** 3 * len(angles)
+    line = {"workload": "backpropagate_3d Rytov, 64x64 detector, 100 angles, "
+                        "sphere r=16 n=1.343, padded to 128x128, 64^3",
+            "fp32_ms": res["fp32"][0] * 1e3,
+            "fp32_value": V / res["fp32"][0],
+            "fp64_ms": res["fp64"][0] * 1e3,
+            "fp64_value": V / res["fp64"][0]}
+    if cpu:
+        cores = cpu_cores()
+        t0 = time.perf_counter()
+        ref = odt_oracle.backpropagate_3d(uB, angles, RES, NM,
+                                          num_cores=cores, **kw)
+        t_cpu = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": V / t_cpu, "s": t_cpu,
+                                "cores": cores, "kind": "port",
+                                "sample": "oracle float64, whole call"}
+        for name in ("fp32", "fp64"):
+            g = res[name][1].astype(np.complex128)
+            line[name + "_rel_l2_vs_oracle"] = float(
+                np.linalg.norm(g - ref) / np.linalg.norm(ref))
+            line[name + "_max_abs_vs_oracle"] = float(np.abs(g - ref).max())
+    return line
 
 
 def batch_bench(steps, warmup):
@@ -390,7 +497,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-sample-angles", type=int, default=36,
+                    help="angles of the bounded CPU-baseline sample")
+    ap.add_argument("--cpu-sample-only", action="store_true",
+                    help="report the bounded sample only (no full oracle run, "
+                         "no error against the oracle)")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 64^2 x 100 workload block")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--batch", action="store_true",
                     help="run the batched 48-sinogram workload instead")
@@ -468,9 +581,11 @@ def main():
     value = V * N_ANG / t_step
 
     # Rotate-accumulate time from HIP events on the plan's stream.  The stage
-    # runs as several launches (angle groups); per launch, the algorithmic
-    # HBM bytes are the coefficient slabs of its angles (complex64, 8 B per
-    # voxel and angle) plus one write of the 256^3 complex64 running sums.
+    # runs as several launches (angle groups).  Algorithmic HBM bytes per
+    # voxel and angle: the complex64 coefficient sample read once (8 B) plus
+    # the final complex64 volume written once per reconstruction, spread
+    # over the angles (8/A B); a launch of B angles is credited with
+    # B * V * (8 + 8/A) bytes.
     plan = _core.last_plan
     plan.enable_timing(True)
     rot, tot = [], []
@@ -484,7 +599,7 @@ def main():
     rot_stage_ms = float(np.mean(rot))
     rot_ms = rot_stage_ms / launches
     ang_per_launch = N_ANG / launches
-    alg_bytes = ang_per_launch * V * 8 + V * 8
+    alg_bytes = ang_per_launch * V * (8 + 8 / N_ANG)
     achieved = alg_bytes / (rot_ms * 1e-3) / 1e9
     log(f"bench: fp32 {t_step * 1e3:.1f} ms/step, rotate {rot_stage_ms:.2f} ms "
         f"in {launches} launches, plan total {np.mean(tot):.1f} ms")
@@ -523,7 +638,7 @@ def main():
                 "algorithmic_bytes": 16 * V}
     del ri
 
-    fp64 = None
+    fp64, out64 = None, None
     if not args.no_fp64:
         log("bench: fp64 path")
         out64 = step(np.float64)
@@ -542,14 +657,33 @@ def main():
                 "fp32_rel_l2_vs_fp64": err, "fp32_max_abs_vs_fp64": maxabs}
         log(f"bench: fp64 {t64 * 1e3:.1f} ms/step, fp32 vs fp64 rel_l2 "
             f"{err:.2e}")
-        del out64
 
-    cpu = None
+    c1 = None
+    if not args.no_c1:
+        log("bench: 64^2 x 100 workload")
+        c1 = c1_workload(cpu=not args.no_cpu_baseline)
+
+    cpu, accuracy = None, None
     if not args.no_cpu_baseline:
-        log("bench: cpu baseline (oracle, bounded sample)")
-        cpu = cpu_baseline(uB, angles, args.cpu_seconds)
+        log("bench: cpu baseline (oracle on all host cores)")
+        cpu, ref = cpu_baseline(uB, angles, args.cpu_sample_angles,
+                                full=not args.cpu_sample_only)
         log(f"bench: cpu {cpu['value']:.3e} voxel-angle/s ({cpu['sample']})")
+        if ref is not None:
+            accuracy = {}
+            for name, g in (("fp32", out32), ("fp64", out64)):
+                if g is None:
+                    continue
+                g = g.cpu().numpy().astype(np.complex128)
+                accuracy[name + "_rel_l2_vs_oracle"] = float(
+                    np.linalg.norm(g - ref) / np.linalg.norm(ref))
+                accuracy[name + "_max_abs_vs_oracle"] = float(
+                    np.abs(g - ref).max())
+            accuracy["oracle_max_abs"] = float(np.abs(ref).max())
+            log(f"bench: vs oracle {accuracy}")
+            del ref
 
+    traffic, traffic_src = traffic_from_profile()
     line = {
         "metric": METRIC,
         "value": value,
@@ -570,12 +704,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(),
+                     "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_rotate_lds2", "ms_per_launch": rot_ms,
                      "launches_per_step": launches,
                      "angles_per_launch": ang_per_launch,
+                     "bytes_per_voxel_angle": 8 + 8 / N_ANG,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "accuracy_vs_oracle": accuracy,
+        "c1": c1,
         "fp64_gpu": fp64,
         "prologue": prologue,
         "epilogue": epilogue,

@@ -465,3 +465,48 @@ def backpropagate_2d(uSin, angles, res, nm, lD=0, weight_angles=True,
                 sino_i.imag, -angles[i] * 180 / np.pi, reshape=False,
                 mode="constant", cval=0)
     return outarr
+
+
+# --------------------------------------------------------------------------
+# angle-parallel evaluation (test infrastructure, no reference counterpart)
+# --------------------------------------------------------------------------
+
+_ap_job = {}
+
+
+def _ap_worker(rng):
+    func, args, kw = _ap_job["job"]
+    return func(*args, angle_range=rng, **kw)
+
+
+def angle_parallel(func, uSin, angles, *args, workers=1, **kw):
+    """Evaluate one of the three restated functions with its angle loop split
+    into ``workers`` contiguous blocks run in a fork process pool, and return
+    the sum of the partial volumes in block order.
+
+    Every function above builds its filters and angle weights from all
+    angles before the loop and then adds one rotated slab per angle, so the
+    partial volume of ``angle_range=(a0, a1)`` is exactly the sum of those
+    angles' contributions; only the order of the floating-point additions
+    differs from the sequential loop (rounding level).  Inner pools are
+    disabled (``num_cores=1`` where the function takes it)."""
+    A = len(angles)
+    workers = max(1, min(int(workers), A))
+    bounds = np.linspace(0, A, workers + 1).round().astype(int)
+    ranges = [(int(a), int(b)) for a, b in zip(bounds[:-1], bounds[1:])
+              if b > a]
+    if func is not backpropagate_2d:
+        kw = dict(kw, num_cores=1)
+    if workers == 1:
+        return func(uSin, angles, *args, **kw)
+    _ap_job["job"] = (func, (uSin, angles) + tuple(args), kw)
+    ctx = mp.get_context("fork")
+    try:
+        with ctx.Pool(processes=len(ranges)) as pool:
+            parts = pool.map(_ap_worker, ranges, chunksize=1)
+    finally:
+        _ap_job.clear()
+    out = parts[0]
+    for p in parts[1:]:
+        out += p
+    return out
