@@ -115,6 +115,21 @@ int odt_plan_execute(odt_plan *plan, const void *d_usin, void *d_out,
 
 int odt_plan_destroy(odt_plan *plan);
 
+/* Allocation-free variant of odt_plan_execute.  There is no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The workspace is a static device array of ODT_SCRATCH_BYTES per GPU,
+ * held by the library until the process ends; plans whose
+ * odt_plan_workspace_size exceeds it are rejected with ODT_ERR_WORKSPACE
+ * (create them with a smaller slab_budget_bytes).  The call allocates and
+ * frees nothing.  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (different streams or threads) race and must
+ * be serialised by the caller.  Must be called with the plan's device
+ * current; timing must be disabled.  Same results as odt_plan_execute. */
+#define ODT_SCRATCH_BYTES (128u << 20)
+int odt_plan_execute_noalloc(odt_plan *plan, const void *d_usin, void *d_out,
+                             void *stream);
+
 /* Kernel timing of the last execute (milliseconds, HIP events on the plan's
  * stream), for the rotate-accumulate stage and the whole call, and the number
  * of rotate-accumulate kernel launches the stage made.  Timing is recorded

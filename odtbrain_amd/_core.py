@@ -132,6 +132,19 @@ class Plan:
         return d_out
 
 
+    def execute_noalloc(self, d_usin, d_out, stream=None):
+        """odt_plan_execute_noalloc: the workspace is the library's static
+        per-GPU scratch (N.ODT_SCRATCH_BYTES); nothing is allocated."""
+        torch = _torch()
+        if stream is None:
+            stream = torch.cuda.current_stream(d_usin.device).cuda_stream
+        N.check(N.lib().odt_plan_execute_noalloc(
+            self.handle, ctypes.c_void_p(d_usin.data_ptr()),
+            ctypes.c_void_p(d_out.data_ptr()), ctypes.c_void_p(stream)),
+            "odt_plan_execute_noalloc")
+        return d_out
+
+
 def slab_budget(per_angle_bytes, A, device):
     """Bytes allowed for the batch of coefficient slabs."""
     env = os.environ.get("ODTBRAIN_AMD_SLAB_BUDGET")

@@ -14,10 +14,11 @@ ODT_KIND_3D, ODT_KIND_3D_TILTED, ODT_KIND_2D = 0, 1, 2
 ODT_FP32, ODT_FP64 = 0, 1
 ODT_PAD_NONE, ODT_PAD_EDGE, ODT_PAD_RAMP = 0, 1, 2
 ODT_FLAG_IIR_PREFILTER = 1
+ODT_SCRATCH_BYTES = 128 << 20  # static workspace of odt_plan_execute_noalloc
 
 EXPORTED = (
     "odt_plan_create", "odt_plan_workspace_size", "odt_plan_execute",
-    "odt_plan_destroy", "odt_plan_enable_timing", "odt_plan_last_timing",
+    "odt_plan_execute_noalloc", "odt_plan_destroy", "odt_plan_enable_timing", "odt_plan_last_timing",
     "odt_fft_c2c", "odt_spline_prefilter", "odt_rotate_accumulate",
     "odt_plan_adjoint",
     "odt_batch_create", "odt_batch_launch", "odt_batch_destroy",
@@ -79,6 +80,7 @@ def lib():
     L.odt_plan_create.argtypes = [ctypes.POINTER(OdtDesc), ctypes.POINTER(vp)]
     L.odt_plan_workspace_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.odt_plan_execute.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.odt_plan_execute_noalloc.argtypes = [vp, vp, vp, vp]
     L.odt_plan_destroy.argtypes = [vp]
     L.odt_plan_enable_timing.argtypes = [vp, ctypes.c_int]
     L.odt_plan_last_timing.argtypes = [vp, dp, dp, dp,

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -116,6 +117,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 struct odt_plan {
   int kind, precision, order, onlyreal, flags, pstride, A;
+  int device = 0;  // HIP device the plan's tables live on
   double padval_re, padval_im;
   Geom g;
   bool explicit_iir;
@@ -454,6 +456,34 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
   return ODT_OK;
 }
 
+// Workspace of the allocation-free entry point (odt_plan_execute_noalloc):
+// a file-scope device array, one instance per GPU the library's code object
+// is loaded on, held until the process ends.  Host code never uses the
+// array's name as an address: its device address is looked up once per
+// device with hipGetSymbolAddress and kept in scratch_ptr[].
+static __device__ __align__(256) unsigned char scratch[ODT_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `dev` (the current device), looked up on
+// the first call for that device.
+int scratch_address(int dev, void **out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(ODT_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(ODT_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    scratch_ptr[dev] = ptr;
+  }
+  *out = scratch_ptr[dev];
+  return ODT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -492,7 +522,10 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   if (2 * (size_t)d->lNx * esz > max_lds || 2 * (size_t)d->lNy * esz > max_lds)
     return fail(ODT_ERR_UNSUPPORTED, "transform length exceeds the LDS capacity");
 
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   odt_plan *p = new odt_plan();
+  p->device = dev;
   p->kind = d->kind;
   p->precision = d->precision;
   p->order = d->intp_order;
@@ -647,6 +680,34 @@ int odt_plan_execute(odt_plan *p, const void *d_usin, void *d_out, void *ws,
   if (ws_bytes < p->ws_bytes || (!ws && p->ws_bytes))
     return fail(ODT_ERR_WORKSPACE, "workspace too small: need " +
                                        std::to_string(p->ws_bytes) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  char *w = (char *)ws;
+  if (p->precision == ODT_FP64)
+    return p->onlyreal ? execute_t<double, double>(p, d_usin, d_out, w, st)
+                       : execute_t<double, cpx<double>>(p, d_usin, d_out, w, st);
+  return p->onlyreal ? execute_t<float, float>(p, d_usin, d_out, w, st)
+                     : execute_t<float, cpx<float>>(p, d_usin, d_out, w, st);
+}
+
+int odt_plan_execute_noalloc(odt_plan *p, const void *d_usin, void *d_out,
+                             void *stream) {
+  if (!p || !d_usin || !d_out) return fail(ODT_ERR_INVALID, "null argument");
+  if (p->timing)
+    return fail(ODT_ERR_INVALID, "odt_plan_execute_noalloc: timing enabled on the plan");
+  if (p->ws_bytes > (size_t)ODT_SCRATCH_BYTES)
+    return fail(ODT_ERR_WORKSPACE,
+                "odt_plan_execute_noalloc: the plan needs " + std::to_string(p->ws_bytes) +
+                    " bytes of workspace, the static scratch holds " +
+                    std::to_string((size_t)ODT_SCRATCH_BYTES) +
+                    " (create the plan with a smaller slab_budget_bytes)");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != p->device)
+    return fail(ODT_ERR_INVALID, "odt_plan_execute_noalloc: the current device is not "
+                                 "the plan's device");
+  void *ws = nullptr;
+  const int rc = scratch_address(dev, &ws);
+  if (rc != ODT_OK) return rc;
   hipStream_t st = (hipStream_t)stream;
   char *w = (char *)ws;
   if (p->precision == ODT_FP64)
