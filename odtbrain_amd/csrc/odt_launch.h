@@ -32,13 +32,16 @@ struct PropAux {
 
 template <typename R>
 hipError_t launch_twiddles(cpx<R> *tw, int n, hipStream_t st);
-template <typename R>
-hipError_t launch_fwd_rows(const cpx<R> *usin, const double *w, cpx<R> *F1,
-                           const cpx<R> *twx, const Geom &g, double pr,
+// Forward stage: pad, transforms and prefactor computed in C (float64 for
+// both precisions where the transform fits the LDS in float64), the support
+// spectrum S stored in R.
+template <typename RI, typename C>
+hipError_t launch_fwd_rows(const cpx<RI> *usin, const double *w, cpx<C> *F1,
+                           const cpx<C> *twx, const Geom &g, double pr,
                            double pi, int a0, int na, int nb, hipStream_t st);
-template <typename R>
-hipError_t launch_fwd_cols(const cpx<R> *F1, cpx<R> *S, const cpx<R> *twy,
-                           const cpx<R> *P, const Geom &g, int na, int nb,
+template <typename RI, typename C>
+hipError_t launch_fwd_cols(const cpx<C> *F1, cpx<RI> *S, const cpx<C> *twy,
+                           const cpx<C> *P, const Geom &g, int na, int nb,
                            hipStream_t st);
 template <typename R>
 hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,

@@ -128,6 +128,11 @@ struct odt_plan {
   size_t esz_c, esz_v;        // bytes of complex element / slab element
   size_t off_F1, off_S, off_T, off_slab, off_comp, off_win, off_part, ws_bytes;
   int RGG = 1;  // concurrent angle groups of the 2D rotate-accumulate
+  // forward stage in float64 (always for FP64 plans; for FP32 plans when
+  // the transforms fit the LDS in float64); the float64 twiddles and
+  // prefactor it uses (the same tables as twx/twy/P in FP64 plans)
+  bool fwd64 = true;
+  void *twx64 = nullptr, *twy64 = nullptr, *P64 = nullptr;
   // device tables
   void *twx = nullptr, *twy = nullptr, *P = nullptr, *aux = nullptr;
   void *q0x = nullptr, *q0y = nullptr, *q0z = nullptr;
@@ -160,6 +165,8 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
       std::complex<double>(0.0, -1.0) * km / two_pi * dphi0;
   const double norm = 1.0 / ((double)g.lNx * (double)g.lNy);
   std::vector<cpx<R>> P((size_t)Lx * Ly);
+  const bool sep64 = p->fwd64 && sizeof(R) != sizeof(double);
+  std::vector<cpx<double>> P64(sep64 ? (size_t)Lx * Ly : 0);
   std::vector<double> pa((size_t)Lx * Ly);
   std::vector<PropAux<R>> aux(g.fuse_z ? (size_t)Lx * Ly : 1);
   const double valx = 1.0 / (double)g.lNx, valy = 1.0 / (double)g.lNy;
@@ -180,6 +187,7 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
       if (g.fuse_y) pf /= b0 + 2.0 * b1 * std::cos(two_pi * jy * valy);
       const size_t t = (size_t)c * Ly + cy;
       P[t] = odt::mk<R>((R)pf.real(), (R)pf.imag());
+      if (sep64) P64[t] = odt::mk<double>(pf.real(), pf.imag());
       const double a = km * (M - 1.0);
       pa[t] = a;
       if (g.fuse_z) {
@@ -224,6 +232,19 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   HIPCHK(hipMalloc(&p->twy, (size_t)g.lNy * sizeof(cpx<R>)), "hipMalloc(twy)");
   HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twx, g.lNx, 0), "twiddles x");
   HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twy, g.lNy, 0), "twiddles y");
+  if (sep64) {
+    HIPCHK(hipMalloc(&p->P64, P64.size() * sizeof(cpx<double>)), "hipMalloc(P64)");
+    HIPCHK(hipMemcpy(p->P64, P64.data(), P64.size() * sizeof(cpx<double>),
+                     hipMemcpyHostToDevice), "copy P64");
+    HIPCHK(hipMalloc(&p->twx64, (size_t)g.lNx * sizeof(cpx<double>)), "hipMalloc(twx64)");
+    HIPCHK(hipMalloc(&p->twy64, (size_t)g.lNy * sizeof(cpx<double>)), "hipMalloc(twy64)");
+    HIPCHK(odt::launch_twiddles<double>((cpx<double> *)p->twx64, g.lNx, 0), "twiddles x64");
+    HIPCHK(odt::launch_twiddles<double>((cpx<double> *)p->twy64, g.lNy, 0), "twiddles y64");
+  } else if (p->fwd64) {
+    p->twx64 = p->twx;
+    p->twy64 = p->twy;
+    p->P64 = p->P;
+  }
   // The column stage's propagator (times the fused z-prefilter terms) is
   // the same for every angle: tabulated once per plan when it fits in
   // 2 GiB (C2: 256 x 227 x 227 complex64, 106 MB).
@@ -250,6 +271,9 @@ void free_plan(odt_plan *p) {
                   p->prop_a, p->weights, p->params, p->fir_taps, p->gtab};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
+  if (p->P64 && p->P64 != p->P) (void)hipFree(p->P64);
+  if (p->twx64 && p->twx64 != p->twx) (void)hipFree(p->twx64);
+  if (p->twy64 && p->twy64 != p->twy) (void)hipFree(p->twy64);
   for (auto e : p->ev_rot) (void)hipEventDestroy(e);
   if (p->ev_t0) (void)hipEventDestroy(p->ev_t0);
   if (p->ev_t1) (void)hipEventDestroy(p->ev_t1);
@@ -275,12 +299,26 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
     const int nb = std::min(p->NB, p->A - b0);
     for (int c0 = b0; c0 < b0 + nb; c0 += p->NC) {
       const int n = std::min(p->NC, b0 + nb - c0);
-      HIPCHK(odt::launch_fwd_rows<R>(usin, p->weights, F1, (const cpx<R> *)p->twx, g,
-                                     p->padval_re, p->padval_im, c0, n, p->nb_fr, st),
-             "k_fwd_rows");
-      HIPCHK(odt::launch_fwd_cols<R>(F1, S, (const cpx<R> *)p->twy, (const cpx<R> *)p->P,
-                                     g, n, p->nb_fc, st),
-             "k_fwd_cols");
+      if (p->fwd64) {
+        cpx<double> *F1d = (cpx<double> *)(ws + p->off_F1);
+        HIPCHK((odt::launch_fwd_rows<R, double>(usin, p->weights, F1d,
+                                                (const cpx<double> *)p->twx64, g,
+                                                p->padval_re, p->padval_im, c0, n,
+                                                p->nb_fr, st)),
+               "k_fwd_rows");
+        HIPCHK((odt::launch_fwd_cols<R, double>(F1d, S, (const cpx<double> *)p->twy64,
+                                                (const cpx<double> *)p->P64, g, n,
+                                                p->nb_fc, st)),
+               "k_fwd_cols");
+      } else {
+        HIPCHK((odt::launch_fwd_rows<R, R>(usin, p->weights, F1, (const cpx<R> *)p->twx, g,
+                                           p->padval_re, p->padval_im, c0, n, p->nb_fr,
+                                           st)),
+               "k_fwd_rows");
+        HIPCHK((odt::launch_fwd_cols<R, R>(F1, S, (const cpx<R> *)p->twy,
+                                           (const cpx<R> *)p->P, g, n, p->nb_fc, st)),
+               "k_fwd_cols");
+      }
       HIPCHK(odt::launch_col_ifft<R>(S, T, (const cpx<R> *)p->twy, p->prop_a,
                                      (const PropAux<R> *)p->aux, (const R *)p->q0z,
                                      (const R *)p->q0y, g, n, p->nb_col, p->ldt,
@@ -583,14 +621,18 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   if (tenv) t_budget = (size_t)std::atoll(tenv);
   p->NC = (int)std::max<size_t>(1, std::min<size_t>(t_budget / t_angle, (size_t)p->NB));
   p->NC = std::min(p->NC, 64);
-  p->nb_fr = per_block(g.lNx, esz, g.lNy);
-  p->nb_fc = per_block(g.lNy, esz, g.Lx);
+  // float64 forward stage where its transforms fit the LDS in float64
+  p->fwd64 = d->precision == ODT_FP64 ||
+             (2 * (size_t)d->lNx * 16 <= max_lds && 2 * (size_t)d->lNy * 16 <= max_lds);
+  const size_t esz_f = p->fwd64 ? 16 : esz;
+  p->nb_fr = per_block(g.lNx, esz_f, g.lNy);
+  p->nb_fc = per_block(g.lNy, esz_f, g.Lx);
   p->nb_col = per_block(g.lNy, esz, g.ln);
   p->nb_row = per_block(g.lNx, esz, g.Ny);
 
   size_t off = 0;
   p->off_F1 = off;
-  off = align256(off + (size_t)p->NC * g.Lx * g.lNy * esz);
+  off = align256(off + (size_t)p->NC * g.Lx * g.lNy * esz_f);
   p->off_S = off;
   off = align256(off + (size_t)p->NC * g.Lx * g.Ly * esz);
   p->off_T = off;
