@@ -227,8 +227,9 @@ int odt_sinogram_prepare(int precision, int ndim, int32_t A, int32_t Ny,
 /* Missing-apple-core correction (odtbrain/apple.py:124-262, correct) of an
  * object function f of shape (n0, n1, n2) with n0 == n2 (the y axis n1 is
  * the axis of symmetry): iterative projections with the non-negativity
- * constraint (method 0; method 1, the symmetric-histogram constraint, is
- * not provided and returns ODT_ERR_UNSUPPORTED), computed in complex128.
+ * constraint (method 0, apple.py:52-71) or the symmetric-histogram
+ * constraint (method 1, apple.py:74-121, with its zero-frequency update
+ * apple.py:235-237), computed in complex128.
  * d_f complex (f_precision), d_out complex (out_precision; the reference
  * returns complex64).  d_mask: null or n0*n1*n2 bytes (nonzero = may be
  * set to zero).  bg_shell_width < 0: no background shell.
@@ -248,6 +249,17 @@ int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2,
                       double min_diff, int out_precision, void *d_out,
                       void *d_ws, size_t ws_bytes, void *stream,
                       int32_t *iterations);
+
+/* Envelope of the Fourier amplitudes outside the apple core
+ * (odtbrain/apple.py:288-353, envelope_gauss): |F| with the core filled
+ * from the (1,0,2) transpose and, where that leaves gaps, from the nearest
+ * valid sample, Gaussian-filtered (sigma = max(shape)/100, truncate 4,
+ * zero outside).  d_ftdata complex128, d_core n0*n1*n2 bytes, d_env
+ * float64; n0 == n2; workspace from odt_apple_workspace_size;
+ * asynchronous. */
+int odt_apple_envelope(int32_t n0, int32_t n1, int32_t n2, const void *d_ftdata,
+                       const uint8_t *d_core, void *d_env, void *d_ws, size_t ws_bytes,
+                       void *stream);
 
 const char *odt_last_error(void);
 int odt_version(void);

@@ -25,6 +25,7 @@ EXPORTED = (
     "odt_odt_to_ri", "odt_opt_to_ri",
     "odt_prepare_workspace_size", "odt_sinogram_prepare",
     "odt_apple_workspace_size", "odt_apple_core", "odt_apple_correct",
+    "odt_apple_envelope",
     "odt_last_error", "odt_version",
 )
 
@@ -116,6 +117,8 @@ def lib():
         ctypes.c_int, vp, ctypes.c_double, ctypes.c_double, i32,
         ctypes.c_double, ctypes.c_int, vp, vp, ctypes.c_size_t, vp,
         ctypes.POINTER(i32)]
+    L.odt_apple_envelope.argtypes = [i32, i32, i32, vp, vp, vp, vp,
+                                     ctypes.c_size_t, vp]
     L.odt_last_error.restype = ctypes.c_char_p
     L.odt_last_error.argtypes = []
     L.odt_version.argtypes = []

@@ -12,6 +12,12 @@
 //                     transpose, leftovers from the nearest valid sample,
 //                     separable Gaussian (truncate 4, zero outside)
 //   k_constraint_nn   apple.py:52-71
+//   k_sh_*            symmetric-histogram constraint, apple.py:74-121: range
+//                     of Re f over the spill-over region (and background
+//                     shell), numpy's 100-bin histogram of Re f, the
+//                     half-maximum band around its peak, values outside
+//                     the band halved
+//   k_gauss_weights   scipy.ndimage's normalised Gaussian taps
 //   k_apple_fourier   envelope suppression, measured data outside the core,
 //                     per-block sums of |F| over the core
 //   k_scale           1/N after the inverse transform
@@ -238,6 +244,185 @@ __global__ void __launch_bounds__(256)
   data[i] = mk<double>(re, 0.0);
 }
 
+// Symmetric-histogram constraint (apple.py:74-121), in four steps.
+// Step 1: imaginary parts 0; per block the minimum and maximum of Re f over
+// the spill-over region united with the background shell (bg_width >= 0).
+constexpr int kShBlocks = 1024;
+constexpr int kShBins = 100;
+
+__global__ void __launch_bounds__(256)
+    k_sh_minmax(cpx<double> *data, int n0, int n1, int n2, double bg_width,
+                double *__restrict__ part) {
+  const long long N = (long long)n0 * n1 * n2;
+  const double cx = n0 / 2.0, cy = n1 / 2.0, cz = n2 / 2.0;
+  double lo = INFINITY, hi = -INFINITY;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % n2), b = (int)((i / n2) % n1), a = (int)(i / ((long long)n1 * n2));
+    const double re = data[i].re;
+    data[i] = mk<double>(re, 0.0);
+    bool in = spill_q(a, b, c, cx, cy, cz, 0.0) > 1;
+    if (bg_width >= 0) in = in || spill_q(a, b, c, cx, cy, cz, bg_width) > 1;
+    if (in) {
+      lo = fmin(lo, re);
+      hi = fmax(hi, re);
+    }
+  }
+  __shared__ double sl[256], sh[256];
+  sl[threadIdx.x] = lo;
+  sh[threadIdx.x] = hi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      sl[threadIdx.x] = fmin(sl[threadIdx.x], sl[threadIdx.x + k]);
+      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sl[0];
+    part[2 * blockIdx.x + 1] = sh[0];
+  }
+}
+
+// Step 2 (one block): fmin, fmax into mm[0..1]; the histogram's outer edges
+// into mm[2..3] (numpy widens an empty range by 0.5 on either side); the
+// bin counts cleared.
+__global__ void __launch_bounds__(256)
+    k_sh_range(const double *__restrict__ part, int nparts, double *__restrict__ mm,
+               unsigned *__restrict__ hist) {
+  __shared__ double sl[256], sh[256];
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    lo = fmin(lo, part[2 * i]);
+    hi = fmax(hi, part[2 * i + 1]);
+  }
+  sl[threadIdx.x] = lo;
+  sh[threadIdx.x] = hi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      sl[threadIdx.x] = fmin(sl[threadIdx.x], sl[threadIdx.x + k]);
+      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kShBins) hist[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) {
+    mm[0] = sl[0];
+    mm[1] = sh[0];
+    const bool same = sl[0] == sh[0];
+    mm[2] = same ? sl[0] - 0.5 : sl[0];
+    mm[3] = same ? sh[0] + 0.5 : sh[0];
+  }
+}
+
+// Bin edge i of numpy.linspace(first, last, 101): i * step + first, the last
+// edge exactly `last`.
+__device__ __forceinline__ double sh_edge(int i, double first, double last) {
+#pragma clang fp contract(off)
+  if (i >= kShBins) return last;
+  const double step = (last - first) / (double)kShBins;
+  return (double)i * step + first;
+}
+
+// Step 3: numpy.histogram(Re f, bins=100, range=(first, last)) with numpy's
+// uniform-bin index rule (index from the scaled offset, then corrected
+// against the edges; the last bin includes the right edge).
+__global__ void __launch_bounds__(256)
+    k_sh_hist(const cpx<double> *__restrict__ data, long long N,
+              const double *__restrict__ mm, unsigned *__restrict__ hist) {
+#pragma clang fp contract(off)
+  __shared__ unsigned lh[kShBins];
+  for (int i = threadIdx.x; i < kShBins; i += blockDim.x) lh[i] = 0u;
+  __syncthreads();
+  const double first = mm[2], last = mm[3];
+  const double denom = last - first;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = data[i].re;
+    if (!(v >= first && v <= last)) continue;
+    long long idx = (long long)(((v - first) / denom) * (double)kShBins);
+    if (idx == kShBins) idx -= 1;
+    if (v < sh_edge((int)idx, first, last)) idx -= 1;
+    if (idx != kShBins - 1 && v >= sh_edge((int)idx + 1, first, last)) idx += 1;
+    atomicAdd(&lh[idx], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kShBins; i += blockDim.x)
+    if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
+// Step 4a (one thread): the histogram's peak (first maximum), the band of
+// bins down to half of it on either side as apple.py:97-99 and count_to_half
+// (apple.py:265-277) define it -- the lower side measured against the bin
+// below the peak -- and the bin centres at its ends into mm[4..5].
+__global__ void k_sh_band(const unsigned *__restrict__ hist, double *__restrict__ mm) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int c = 0;
+  for (int i = 1; i < kShBins; ++i)
+    if (hist[i] > hist[c]) c = i;
+  int nlo = 0;
+  if (c > 0) {
+    const double half = (double)hist[c - 1] / 2.0;
+    for (int k = c - 2; k >= 0; --k) {
+      if ((double)hist[k] < half) break;
+      ++nlo;
+    }
+  }
+  int nhi = 0;
+  {
+    const double half = (double)hist[c] / 2.0;
+    for (int k = c + 1; k < kShBins; ++k) {
+      if ((double)hist[k] < half) break;
+      ++nhi;
+    }
+  }
+  const double first = mm[2], last = mm[3];
+  const double de = sh_edge(1, first, last) - sh_edge(0, first, last);
+  mm[4] = sh_edge(c - nlo + 1, first, last) - de / 2;
+  mm[5] = sh_edge(c + nhi + 1, first, last) - de / 2;
+}
+
+// Step 4b: values strictly between fmin and the band's lower centre or
+// between its upper centre and fmax are halved (where the mask allows),
+// then the background shell is halved.
+__global__ void __launch_bounds__(256)
+    k_sh_apply(cpx<double> *data, int n0, int n1, int n2,
+               const unsigned char *__restrict__ mask, double bg_width,
+               const double *__restrict__ mm) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = (long long)n0 * n1 * n2;
+  if (i >= N) return;
+  const double fmin_ = mm[0], fmax_ = mm[1], lo = mm[4], hi = mm[5];
+  double re = data[i].re;
+  bool out = (re > fmin_ && re < lo) || (re > hi && re < fmax_);
+  if (mask && !mask[i]) out = false;
+  if (out) re *= 0.5;
+  if (bg_width >= 0) {
+    const int c = (int)(i % n2), b = (int)((i / n2) % n1), a = (int)(i / ((long long)n1 * n2));
+    const double cx = n0 / 2.0, cy = n1 / 2.0, cz = n2 / 2.0;
+    const bool outer = spill_q(a, b, c, cx, cy, cz, 0.0) > 1;
+    const bool inner = spill_q(a, b, c, cx, cy, cz, bg_width) > 1;
+    if (outer != inner) re *= 0.5;
+  }
+  data[i] = mk<double>(re, 0.0);
+}
+
+// scipy.ndimage's Gaussian taps (_gaussian_kernel1d: exp(-x^2 / (2 sigma^2))
+// for x = -r..r, divided by their sum), one thread, in order.
+__global__ void k_gauss_weights(double *w, int r, double sigma) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0;
+  for (int k = -r; k <= r; ++k) {
+    const double v = exp(-0.5 / (sigma * sigma) * (double)k * (double)k);
+    w[k + r] = v;
+    s += v;
+  }
+  for (int k = 0; k <= 2 * r; ++k) w[k] /= s;
+}
+
 // Fourier step: suppression above the envelope, the measured data outside
 // the core, per-block sums of |F| over the core.  sh: the zero frequency of
 // the measured data becomes the mean of the two first.
@@ -365,6 +550,31 @@ hipError_t launch_constraint_nn(cpx<double> *data, int n0, int n1, int n2,
                                 const unsigned char *mask, double bg_width, hipStream_t st) {
   k_constraint_nn<<<nb256((long long)n0 * n1 * n2), 256, 0, st>>>(data, n0, n1, n2, mask,
                                                                    bg_width);
+  return hipGetLastError();
+}
+
+hipError_t launch_constraint_sh(cpx<double> *data, int n0, int n1, int n2,
+                                const unsigned char *mask, double bg_width, double *part,
+                                double *mm, unsigned *hist, hipStream_t st) {
+  const long long N = (long long)n0 * n1 * n2;
+  k_sh_minmax<<<kShBlocks, 256, 0, st>>>(data, n0, n1, n2, bg_width, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_sh_range<<<1, 256, 0, st>>>(part, kShBlocks, mm, hist);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  k_sh_hist<<<kShBlocks, 256, 0, st>>>(data, N, mm, hist);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  k_sh_band<<<1, 64, 0, st>>>(hist, mm);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  k_sh_apply<<<nb256(N), 256, 0, st>>>(data, n0, n1, n2, mask, bg_width, mm);
+  return hipGetLastError();
+}
+
+size_t sh_part_bytes() { return (size_t)2 * kShBlocks * sizeof(double); }
+size_t sh_hist_bytes() { return (size_t)kShBins * sizeof(unsigned); }
+
+hipError_t launch_gauss_weights(double *w, int r, double sigma, hipStream_t st) {
+  k_gauss_weights<<<1, 64, 0, st>>>(w, r, sigma);
   return hipGetLastError();
 }
 

@@ -121,6 +121,15 @@ hipError_t launch_envelope(const cpx<double> *F, const unsigned char *core, int 
                            double *env, hipStream_t st);
 hipError_t launch_constraint_nn(cpx<double> *data, int n0, int n1, int n2,
                                 const unsigned char *mask, double bg_width, hipStream_t st);
+// symmetric-histogram constraint; part: sh_part_bytes(), mm: 6 doubles,
+// hist: sh_hist_bytes()
+hipError_t launch_constraint_sh(cpx<double> *data, int n0, int n1, int n2,
+                                const unsigned char *mask, double bg_width, double *part,
+                                double *mm, unsigned *hist, hipStream_t st);
+size_t sh_part_bytes();
+size_t sh_hist_bytes();
+// scipy.ndimage Gaussian taps (2 r + 1 doubles), normalised, on the device
+hipError_t launch_gauss_weights(double *w, int r, double sigma, hipStream_t st);
 hipError_t launch_apple_fourier(cpx<double> *F, cpx<double> *F0, const double *env,
                                 const unsigned char *core, long long N, double factor,
                                 int sh, double *part, double *sum, hipStream_t st);

@@ -996,7 +996,7 @@ int odt_sinogram_prepare(int precision, int ndim, int32_t A, int32_t Ny, int32_t
 namespace {
 struct AppleLayout {
   size_t data, ft, ft0, tmp, env, core, hp, hp2, cp, part, sum, nnan, tw0, tw1, tw2, w,
-      total;
+      shpart, mm, hist, total;
   int r;
 };
 
@@ -1029,6 +1029,9 @@ AppleLayout apple_layout(int n0, int n1, int n2) {
   const double sigma = (double)std::max(n0, std::max(n1, n2)) / 100.0;
   L.r = (int)(4.0 * sigma + 0.5);
   L.w = take((size_t)(2 * L.r + 1) * 8);
+  L.shpart = take(odt::sh_part_bytes());
+  L.mm = take(8 * 8);
+  L.hist = take(odt::sh_hist_bytes());
   L.total = o;
   return L;
 }
@@ -1082,8 +1085,8 @@ int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2, const
   if (n0 != n2) return fail(ODT_ERR_INVALID, "`shape[0]` and `shape[2]` must be identical!");
   if (!(enforce_envelope >= 0 && enforce_envelope <= 1))
     return fail(ODT_ERR_INVALID, "`enforce_envelope` must be in interval [0, 1]");
-  if (method != 0)
-    return fail(ODT_ERR_UNSUPPORTED, "only the non-negativity constraint is provided");
+  if (method != 0 && method != 1)
+    return fail(ODT_ERR_INVALID, "method must be 0 (non-negativity) or 1 (symmetric histogram)");
   const int mmax = std::max(n0, std::max(n1, n2));
   if (2 * (size_t)mmax * 16 > 160 * 1024 || n0 > 65535)
     return fail(ODT_ERR_UNSUPPORTED, "volume edge too long for the transforms");
@@ -1110,16 +1113,9 @@ int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2, const
   HIPCHK(fft3(ft0, tmp, n0, n1, n2, -1, tw0, tw1, tw2, st), "fft3");
   const bool use_env = enforce_envelope != 0.0;
   if (use_env) {
-    // scipy.ndimage gaussian weights: exp(-x^2 / (2 sigma^2)) normalised
-    const double sigma = (double)mmax / 100.0;
-    std::vector<double> wt(2 * L.r + 1);
-    double ssum = 0;
-    for (int k = -L.r; k <= L.r; ++k) {
-      wt[k + L.r] = std::exp(-0.5 / (sigma * sigma) * (double)k * (double)k);
-      ssum += wt[k + L.r];
-    }
-    for (auto &v : wt) v /= ssum;
-    HIPCHK(hipMemcpy(w + L.w, wt.data(), wt.size() * 8, hipMemcpyHostToDevice), "copy w");
+    // scipy.ndimage gaussian weights, computed on the stream
+    HIPCHK(odt::launch_gauss_weights((double *)(w + L.w), L.r, (double)mmax / 100.0, st),
+           "k_gauss_weights");
     HIPCHK(hipMemsetAsync(w + L.nnan, 0, 4, st), "memset");
     double *g = (double *)tmp, *g2 = (double *)tmp + N;
     HIPCHK(odt::launch_envelope(ft0, core, n0, n1, n2, (double *)(w + L.hp),
@@ -1139,12 +1135,18 @@ int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2, const
   double prev = state / (double)N, norm = 0;
   int ii = 0;
   for (; ii < max_iter; ++ii) {
-    HIPCHK(odt::launch_constraint_nn(data, n0, n1, n2, d_mask, bg_shell_width, st),
-           "k_constraint_nn");
+    if (method == 0)
+      HIPCHK(odt::launch_constraint_nn(data, n0, n1, n2, d_mask, bg_shell_width, st),
+             "k_constraint_nn");
+    else
+      HIPCHK(odt::launch_constraint_sh(data, n0, n1, n2, d_mask, bg_shell_width,
+                                       (double *)(w + L.shpart), (double *)(w + L.mm),
+                                       (unsigned *)(w + L.hist), st),
+             "k_sh");
     HIPCHK(hipMemcpyAsync(ft, data, N * 16, hipMemcpyDeviceToDevice, st), "copy");
     HIPCHK(fft3(ft, tmp, n0, n1, n2, -1, tw0, tw1, tw2, st), "fft3");
     HIPCHK(odt::launch_apple_fourier(ft, ft0, use_env ? env : nullptr, core, N,
-                                     enforce_envelope, 0, (double *)(w + L.part),
+                                     enforce_envelope, method == 1, (double *)(w + L.part),
                                      (double *)(w + L.sum), st),
            "k_apple_fourier");
     HIPCHK(hipMemcpyAsync(data, ft, N * 16, hipMemcpyDeviceToDevice, st), "copy");
@@ -1163,6 +1165,32 @@ int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2, const
   }
   if (iterations) *iterations = std::min(ii + 1, (int)max_iter);
   HIPCHK(odt::launch_scale_out(data, N, 1.0, out_precision == ODT_FP64, d_out, st), "out");
+  return ODT_OK;
+}
+
+int odt_apple_envelope(int32_t n0, int32_t n1, int32_t n2, const void *d_ftdata,
+                       const uint8_t *d_core, void *d_env, void *d_ws, size_t ws_bytes,
+                       void *stream) {
+  if (n0 < 1 || n1 < 1 || n2 < 1 || !d_ftdata || !d_core || !d_env || !d_ws)
+    return fail(ODT_ERR_INVALID, "bad arguments");
+  if (n0 != n2) return fail(ODT_ERR_INVALID, "`shape[0]` and `shape[2]` must be identical!");
+  const AppleLayout L = apple_layout(n0, n1, n2);
+  if (ws_bytes < L.total)
+    return fail(ODT_ERR_WORKSPACE, "workspace too small: need " + std::to_string(L.total) +
+                                       " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  char *w = (char *)d_ws;
+  const int mmax = std::max(n0, std::max(n1, n2));
+  const long long N = (long long)n0 * n1 * n2;
+  double *g = (double *)(w + L.tmp), *g2 = g + N;
+  HIPCHK(odt::launch_gauss_weights((double *)(w + L.w), L.r, (double)mmax / 100.0, st),
+         "k_gauss_weights");
+  HIPCHK(hipMemsetAsync(w + L.nnan, 0, 4, st), "memset");
+  HIPCHK(odt::launch_envelope((const cpx<double> *)d_ftdata, d_core, n0, n1, n2,
+                              (double *)(w + L.hp), (double *)(w + L.hp2),
+                              (unsigned char *)(w + L.cp), (unsigned *)(w + L.nnan),
+                              (const double *)(w + L.w), L.r, g, g2, (double *)d_env, st),
+         "envelope");
   return ODT_OK;
 }
 

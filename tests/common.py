@@ -138,3 +138,17 @@ def rel_l2(x, ref):
 
 
 ROOT_DIR = pathlib.Path(__file__).resolve().parent.parent
+
+
+def quarter_turn_case():
+    """Sinogram without mirror symmetry in y (random phase on the reference's
+    Gaussian object) and angles whose differences avoid exact quadrants: at
+    multiples of 90 degrees the rotated coordinates of boundary voxels sit
+    exactly on the plane edge, where rounding decides scipy's out-of-bounds
+    test."""
+    sino, angles = create_test_sino_3d(A=12, Nx=18, Ny=18)
+    rng = np.random.default_rng(0)
+    sino = sino * np.exp(1j * 0.3 * rng.standard_normal(sino.shape))
+    # the tilted geometry uses angles relative to the first one: jitter
+    # every angle so that no difference is a whole quadrant
+    return sino, angles + rng.uniform(0.05, 0.2, angles.shape)

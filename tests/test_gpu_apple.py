@@ -101,3 +101,48 @@ def test_correct_non_cubic_with_nearest_fill(shape, res):
     assert np.abs(got - same).max() <= 2e-6 * scale
     ref = AP.correct(f, res, nm, max_iter=20, fft_dtype=np.complex128)
     assert np.abs(got - ref).max() <= 1e-2 * scale
+
+
+@pytest.mark.parametrize("shape,res,nm,kw", [
+    ((16, 16, 16), 3.0, 1.333, {}),
+    ((20, 20, 20), 3.0, 1.333, {"bg_shell_width": 3}),
+    ((18, 22, 18), 2.5, 1.34, {"enforce_envelope": 0}),
+    ((16, 16, 16), 3.0, 1.333, {"mask": "half"}),
+])
+def test_correct_sh_matches_oracle(shape, res, nm, kw):
+    """method="sh" (the symmetric-histogram constraint, apple.py:74-121,
+    and its zero-frequency update, :235-237) against the oracle in
+    complex128; the histogram thresholds are discrete, so both sides must
+    make the same decisions for the values to agree to rounding."""
+    f = _volume(shape, 3 * sum(shape))
+    kw = dict(kw)
+    if kw.get("mask") == "half":
+        m = np.zeros(shape, dtype=bool)
+        m[:, : shape[1] // 2] = True
+        kw["mask"] = m
+    ref = AP.correct(f, res, nm, method="sh", max_iter=30,
+                     fft_dtype=np.complex128, **kw)
+    got = G.apple.correct(f, res, nm, method="sh", max_iter=30, **kw)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 2e-6 * scale, np.abs(got - ref).max()
+    nn = G.apple.correct(f, res, nm, method="nn", max_iter=30, **kw)
+    assert np.abs(nn - got).max() > 1e-3 * scale  # the constraints differ
+
+
+def test_envelope_gauss_shape():
+    """tests/test_apple.py:90-97: non-cubic volume (first and third axis
+    equal)."""
+    shape = (60, 50, 60)
+    core = G.apple.apple_core_3d(shape=shape, res=.1, nm=1)
+    env = G.apple.envelope_gauss(ftdata=np.ones(shape), core=core)
+    assert env.shape == shape and env.dtype == np.float64
+
+
+@pytest.mark.parametrize("shape,res", [((16, 16, 16), 3.0),
+                                       ((20, 14, 20), 3.0)])
+def test_envelope_gauss_matches_oracle(shape, res):
+    ft = np.fft.fftn(_volume(shape, 5).real)
+    core = AP.apple_core_3d(shape, res, 1.333)
+    got = G.apple.envelope_gauss(ft, core)
+    ref = AP.envelope_gauss(ft, core, nearest="min_index")
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()

@@ -15,7 +15,8 @@ import pytest
 
 import odtbrain_amd as G
 from common import (create_test_sino_2d, create_test_sino_3d, cutout,
-                    get_test_parameter_set, load_golden, rel_l2)
+                    get_test_parameter_set, load_golden, quarter_turn_case,
+                    rel_l2)
 from odtbrain_amd import phantoms
 from oracle import odt_oracle as O
 
@@ -299,3 +300,35 @@ def test_large_linearity_and_symmetry():
     fw = G.backpropagate_3d((2.0 * u - 3.0j * v).astype(np.complex64),
                             angles, **kw)
     assert rel_l2(fw, 2.0 * fu - 3.0j * fv) < 1e-5
+
+
+def test_tilted_x_axis_equals_quarter_turned_y_axis():
+    """Quarter-turned projections with the axis along -x reconstruct as the
+    original projections with the y axis (the identity the oracle is pinned
+    by), on the GPU in FP64 to rounding and in FP32 to the FP32 bar, and
+    against the oracle."""
+    sino, angles = quarter_turn_case()
+    p = get_test_parameter_set(1)[0]
+    turned = np.rot90(sino, k=1, axes=(1, 2)).copy()
+    fy = G.backpropagate_3d_tilted(sino, angles, padval="edge",
+                                   tilted_axis=[0, 1, 0], **p)
+    fx = G.backpropagate_3d_tilted(turned, angles, padval="edge",
+                                   tilted_axis=[-1, 0, 0], **p)
+    assert rel_l2(fx, fy) < 1e-12
+    ref = O.backpropagate_3d_tilted(turned, angles, padval="edge",
+                                    tilted_axis=[-1, 0, 0], **p)
+    assert rel_l2(fx, ref) < FP64_REL_L2
+    fx32 = G.backpropagate_3d_tilted(turned, angles, padval="edge",
+                                     tilted_axis=[-1, 0, 0],
+                                     dtype=np.float32, **p)
+    assert rel_l2(fx32, fy) < FP32_REL_L2
+
+
+def test_tilted_counters():
+    """tests/test_counters.py:80-93."""
+    sino, angles = create_test_sino_3d(Nx=10, Ny=10)
+    jmc, jmm = mp.Value("i", 0), mp.Value("i", 0)
+    G.backpropagate_3d_tilted(sino, angles, padval=0, dtype=float,
+                              count=jmc, max_count=jmm,
+                              **get_test_parameter_set(1)[0])
+    assert jmc.value == jmm.value != 0

@@ -9,8 +9,10 @@ import pytest
 import scipy.ndimage as ndi
 import scipy.special
 
-from common import (create_test_sino_2d, create_test_sino_3d, cutout,
-                    get_test_parameter_set, load_golden)
+from common import (create_test_sino_2d, create_test_sino_3d,
+                    create_test_sino_3d_tilted, cutout,
+                    get_test_parameter_set, load_golden, quarter_turn_case)
+from oracle import apple as AP
 from oracle import bspline as B
 from oracle import odt_oracle as O
 
@@ -366,3 +368,88 @@ def test_angle_parallel_equals_sequential(which):
     # a dropped block would show: the first block alone differs
     part = f(s, a, *args, angle_range=(0, 3), **kw)
     assert np.abs(part - seq).max() > 1e-3 * np.abs(seq).max()
+
+
+def test_oracle_tilted_x_axis_equals_quarter_turned_y_axis():
+    """Turning every projection a quarter turn (np.rot90, k=1, detector
+    y -> x) turns the rotation axis from +y to -x; the tilted path rotates
+    that axis back onto the y-z plane (angz) and builds its plane rotation
+    into the resampling, so its reconstruction must equal the y-axis one to
+    float64 rounding.  This pins the x component of the tilted ramp
+    |kx v + ky u| and the angz handling (_alg3d_bppt.py:736-771, 904-936).
+    The antiparallel axis (+x) must not match."""
+    sino, angles = quarter_turn_case()
+    p = get_test_parameter_set(1)[0]
+    fy = O.backpropagate_3d_tilted(sino, angles, padval="edge",
+                                   tilted_axis=[0, 1, 0], **p)
+    turned = np.rot90(sino, k=1, axes=(1, 2)).copy()
+    fx = O.backpropagate_3d_tilted(turned, angles, padval="edge",
+                                   tilted_axis=[-1, 0, 0], **p)
+    assert np.linalg.norm(fx - fy) <= 1e-12 * np.linalg.norm(fy)
+    anti = O.backpropagate_3d_tilted(turned, angles, padval="edge",
+                                     tilted_axis=[1, 0, 0], **p)
+    assert np.linalg.norm(anti - fy) > 1e-2 * np.linalg.norm(fy)
+
+
+def test_oracle_tilted_soft_plane_rotation_and_alignment():
+    """tests/test_alg3d_bppt.py:53-126 (ODTbrain's soft tests, atol/rtol
+    0.2) run on the oracle."""
+    p = get_test_parameter_set(1)[0]
+
+    def rec(angz, onlyreal):
+        sino, angles = create_test_sino_3d_tilted(tilt_plane=angz, A=21)
+        rotmat = np.array([[np.cos(angz), -np.sin(angz), 0],
+                           [np.sin(angz), np.cos(angz), 0],
+                           [0, 0, 1]])
+        return O.backpropagate_3d_tilted(
+            sino, angles, padval="edge", tilted_axis=rotmat @ [0, 1, 0],
+            padding=(False, False), dtype=float, onlyreal=onlyreal, **p)
+
+    res = [cutout(rec(a, False)).flatten().view(float)
+           for a in [1.1, 0.0, 0.234, 2.80922, -.29, 9.87]]
+    for i in range(len(res)):
+        assert np.allclose(res[i], res[i - 1], atol=.2, rtol=.2)
+    res = [rec(a, True) for a in [0, np.pi / 2, np.pi, 3 * np.pi / 2,
+                                   2 * np.pi]]
+    for i in range(len(res)):
+        assert np.allclose(res[i], res[i - 1], atol=.2, rtol=.2)
+
+
+def test_apple_constraint_sh_known_answer():
+    """constraint_sh (apple.py:74-121) on a volume whose histogram is known
+    by construction.  Spill-over region (outside the inscribed ellipsoid):
+    values -1 (and one 1), so fmin = -1, fmax = 1 and the 100 bins are 0.02
+    wide, bin i = [-1 + 0.02 i, -0.98 + 0.02 i).  Inside: the peak bin 50
+    [0, 0.02) (0.005 and 0.009), 2 values in bin 49 (-0.015), 50 in bin 48
+    (-0.035), 10 in bin 47 (-0.055), single values at 0.5 and -0.7.
+    count_to_half on the bins below the peak in reverse measures against
+    bin 49 (2 values): bins 48 and 47 are not below 1, bin 46 (empty) is,
+    so the band starts at bin 48 and its lower end is that bin's centre
+    -0.03; above the peak bin 51 is empty, so the upper end is the peak's
+    centre 0.01.  Values strictly between fmin and -0.03 or strictly
+    between 0.01 and fmax are halved: -0.035, -0.055, -0.7 and 0.5.  The
+    value -0.015 is kept only because of that reference bin: measured
+    against the peak (858 values), bin 49 would already end the band."""
+    shape = (12, 12, 12)
+    spov = AP.spillover_region(shape)
+    inner = np.argwhere(~spov)
+    data = np.zeros(shape, dtype=np.complex128)
+    data.real[spov] = -1.0
+    corner = tuple(np.argwhere(spov)[0])
+    data.real[corner] = 1.0
+    vals = ([0.005] * 60 + [-0.015] * 2 + [-0.035] * 50 + [-0.055] * 10 +
+            [0.5, -0.7])
+    assert len(vals) <= len(inner)
+    rest = [0.009] * (len(inner) - len(vals))  # also in the peak bin
+    for p, v in zip(inner, vals + rest):
+        data.real[tuple(p)] = v
+    data.imag[:] = 0.3
+    before = data.real.copy()
+    AP.constraint_sh(data)
+    assert np.all(data.imag == 0)
+    want = before.copy()
+    for v in (-0.035, -0.055, -0.7, 0.5):
+        want[before == v] = v / 2
+    assert np.array_equal(data.real, want)
+    assert np.all(data.real[before == -0.015] == -0.015)
+    assert np.all(data.real[np.abs(before) == 1] == before[np.abs(before) == 1])
