@@ -146,13 +146,17 @@ class Plan:
 
 
 def slab_budget(per_angle_bytes, A, device):
-    """Bytes allowed for the batch of coefficient slabs."""
+    """Bytes allowed for the batch of coefficient slabs: all angles, up to
+    three eighths of the device's memory.  Derived from the total memory,
+    not the free memory, so that identical calls build identical plans (the
+    budget is part of the plan cache key, and the caching allocator's
+    holdings change the free memory between calls)."""
     env = os.environ.get("ODTBRAIN_AMD_SLAB_BUDGET")
     if env:
         return int(env)
     torch = _torch()
-    free, _total = torch.cuda.mem_get_info(device)
-    return int(max(per_angle_bytes, min(A * per_angle_bytes, 0.5 * free)))
+    total = torch.cuda.get_device_properties(device).total_memory
+    return int(max(per_angle_bytes, min(A * per_angle_bytes, 3 * total // 8)))
 
 
 def get_plan(desc_fields, weights, params, device):

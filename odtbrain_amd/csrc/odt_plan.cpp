@@ -138,7 +138,7 @@ struct odt_plan {
   void *q0x = nullptr, *q0y = nullptr, *q0z = nullptr;
   double *prop_a = nullptr, *weights = nullptr, *params = nullptr;
   void *gtab = nullptr;      // propagator table of the column stage (or null)
-  void *fir_taps = nullptr;  // adjoint: prefilter impulse response (lazily)
+  void *fir_taps = nullptr;  // adjoint: prefilter impulse response
   int fir_K = 0;
   // timing
   bool timing = false;
@@ -419,6 +419,22 @@ std::vector<double> bspline_fir(int order, int *K) {
   return std::vector<double>(h.begin() + (kc - kk), h.begin() + (kc + kk + 1));
 }
 
+// Prefilter impulse response of the adjoint's transposed prefilter, uploaded
+// at plan creation (so that odt_plan_adjoint only launches kernels).
+template <typename R>
+int upload_fir(odt_plan *p) {
+  int K = 0;
+  const std::vector<double> h = bspline_fir(p->order, &K);
+  const int pad = odt::adj_fir_pad();
+  std::vector<R> hz(2 * K + 2 * pad, R(0));
+  for (int k = -K; k <= K; ++k) hz[k + K + pad] = (R)h[k + K];
+  HIPCHK(hipMalloc(&p->fir_taps, hz.size() * sizeof(R)), "hipMalloc(taps)");
+  HIPCHK(hipMemcpy(p->fir_taps, hz.data(), hz.size() * sizeof(R), hipMemcpyHostToDevice),
+         "hipMemcpy(taps)");
+  p->fir_K = K;
+  return ODT_OK;
+}
+
 // Adjoint of execute_t for the y-axis reconstruction (see odt_adjoint.hip).
 template <typename R>
 int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
@@ -438,18 +454,8 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
   const bool fir = p->NB >= 2;
   const int step = fir ? p->NB / 2 : p->NB;
   cpx<R> *slabB = slab + (size_t)step * per_angle;
-  if (fir && !p->fir_taps) {
-    int K = 0;
-    const std::vector<double> h = bspline_fir(p->order, &K);
-    const int pad = odt::adj_fir_pad();
-    std::vector<R> hz(2 * K + 2 * pad, R(0));
-    for (int k = -K; k <= K; ++k) hz[k + K + pad] = (R)h[k + K];
-    HIPCHK(hipMalloc(&p->fir_taps, hz.size() * sizeof(R)), "hipMalloc(taps)");
-    HIPCHK(hipMemcpy(p->fir_taps, hz.data(), hz.size() * sizeof(R),
-                     hipMemcpyHostToDevice),
-           "hipMemcpy(taps)");
-    p->fir_K = K;
-  }
+  if (fir && !p->fir_taps)
+    return fail(ODT_ERR_UNSUPPORTED, "adjoint: the plan has no prefilter taps");
   for (int b0 = 0; b0 < p->A; b0 += step) {
     const int nb = std::min(step, p->A - b0);
     HIPCHK(odt::launch_adj_rotate<R>(p->order, gout, nb, p->params + 6 * (size_t)b0,
@@ -610,13 +616,15 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   // column stage and read back by the row stage.  Larger chunks measured
   // faster at 256^2 x 360 (T budget 1 GiB: 90.8 ms per reconstruction,
   // 4 GiB: 87.4, 8 GiB: 86.9; smaller than 1 GiB: slower), so up to 8 GiB,
-  // at most an eighth of the free memory.  Tuning knob (bytes).
+  // at most a 32nd of the device memory.  Tuning knob (bytes).
   const char *tenv = std::getenv("ODTBRAIN_AMD_T_BUDGET");
   size_t t_budget = (size_t)8 << 30;
   {
+    // from the total memory (not the free memory): identical descriptions
+    // give identical plans
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
-      t_budget = std::min(t_budget, std::max(free_b / 8, (size_t)1 << 30));
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+      t_budget = std::min(t_budget, std::max(total_b / 32, (size_t)1 << 30));
   }
   if (tenv) t_budget = (size_t)std::atoll(tenv);
   p->NC = (int)std::max<size_t>(1, std::min<size_t>(t_budget / t_angle, (size_t)p->NB));
@@ -695,6 +703,10 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
     return hip_fail(e, "plan upload");
   }
   rc = d->precision == ODT_FP64 ? upload_tables<double>(p, d) : upload_tables<float>(p, d);
+  // plans the adjoint supports get its prefilter taps now
+  if (rc == ODT_OK && p->kind == ODT_KIND_3D && !p->onlyreal && p->explicit_iir &&
+      p->order >= 2)
+    rc = d->precision == ODT_FP64 ? upload_fir<double>(p) : upload_fir<float>(p);
   if (rc != ODT_OK) {
     std::string keep = g_err;
     free_plan(p);

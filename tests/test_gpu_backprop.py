@@ -332,3 +332,35 @@ def test_tilted_counters():
                               count=jmc, max_count=jmm,
                               **get_test_parameter_set(1)[0])
     assert jmc.value == jmm.value != 0
+
+
+@pytest.mark.parametrize("batch_angles,groups", [(20, None), (21, None),
+                                                 (42, 3), (22, 2)])
+def test_2d_angle_groups_across_slab_batches(monkeypatch, batch_angles,
+                                             groups):
+    """2D plans whose slab budget splits the 43 angles into batches: 20
+    angles per batch (20, 20, 3), 21 (21, 21 and a single-angle last batch,
+    which takes the ungrouped kernel), the group count overridden
+    (ODTBRAIN_AMD_ROTATE_2D_GROUPS, read when a plan is built: each case
+    has its own budget, hence its own plan).  Between batches the group fold reads
+    and writes the running sum and its compensation; FP64 and FP32 against
+    the oracle and against the single-batch run."""
+    sino, angles = create_test_sino_2d(A=43, N=64, ampl_range=(0.9, 1.1))
+    kw = dict(res=3.3, nm=1.34, lD=1.0)
+    ref = O.backpropagate_2d(sino, angles, **kw)
+    whole64 = G.backpropagate_2d(sino, angles, **kw)
+    whole32 = G.backpropagate_2d(sino, angles, dtype=np.float32, **kw)
+    for dt, esz in ((np.float64, 16), (np.float32, 8)):
+        monkeypatch.setenv("ODTBRAIN_AMD_SLAB_BUDGET",
+                           str(batch_angles * 64 * 64 * esz))
+        if groups is not None:
+            monkeypatch.setenv("ODTBRAIN_AMD_ROTATE_2D_GROUPS", str(groups))
+        got = G.backpropagate_2d(sino, angles, dtype=dt, **kw)
+        monkeypatch.delenv("ODTBRAIN_AMD_SLAB_BUDGET")
+        monkeypatch.delenv("ODTBRAIN_AMD_ROTATE_2D_GROUPS", raising=False)
+        if dt is np.float64:
+            assert rel_l2(got, ref) < FP64_REL_L2
+            assert rel_l2(got, whole64) < 1e-13
+        else:
+            assert rel_l2(got, ref) < FP32_REL_L2
+            assert rel_l2(got, whole32) < 1e-6
