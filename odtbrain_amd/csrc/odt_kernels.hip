@@ -1506,6 +1506,199 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged affine gather-accumulate for the tilted axis (same arithmetic
+// as k_affine_acc).  A block (4 waves) owns an 8x8x8 output tile; wave w
+// holds the (x, z) patch of the planes y0 + 2w and y0 + 2w + 1, lane l the
+// voxel (x0 + (l & 7), z0 + (l >> 3)) of both.  Per angle the block copies
+// the window of the slab that holds every tap (mirrored taps included) of
+// the tile's in-range voxels into LDS -- at most WIN = 15 + ORDER samples
+// per axis: a rotation moves a 7-sample tile edge over at most 7*sqrt(3)
+// input samples along any axis -- and every thread gathers its taps from
+// there.  k_affine_windows gives each (angle, tile) its window; blocks are
+// numbered so that each XCD takes a contiguous run of tiles (neighbouring
+// windows overlap and meet in that XCD's L2).
+// ---------------------------------------------------------------------------
+constexpr int kAffTile = 8;
+
+template <int ORDER>
+__host__ __device__ constexpr int aff_win() { return 15 + ORDER; }
+
+size_t affine_window_entries(int ny, int nz, int nx) {
+  return (size_t)((nx + kAffTile - 1) / kAffTile) * ((ny + kAffTile - 1) / kAffTile) *
+         ((nz + kAffTile - 1) / kAffTile);
+}
+
+// Window rows [lo, hi] along one axis (edge n) for coordinates spanning
+// [clo, chi]: the taps start(c) .. start(c) + ORDER of the in-range part,
+// after one whole-sample reflection at either end; packed lo | ext << 16,
+// ext 0 when no coordinate is in range.
+template <int ORDER>
+__device__ __forceinline__ int aff_axis(double clo, double chi, int n) {
+  clo = fmax(clo, 0.0);
+  chi = fmin(chi, (double)(n - 1));
+  if (!(clo <= chi)) return 0;
+  const int s0 = spline_start<ORDER>(clo), s1 = spline_start<ORDER>(chi) + ORDER;
+  int lo = s0 < 2 * n - 2 - s1 ? s0 : 2 * n - 2 - s1;
+  int hi = s1 > -s0 ? s1 : -s0;
+  lo = lo < 0 ? 0 : lo;
+  hi = hi > n - 1 ? n - 1 : hi;
+  int ext = hi - lo + 1;
+  ext = ext < aff_win<ORDER>() ? ext : aff_win<ORDER>();
+  return lo | (ext << 16);
+}
+
+// Window of every (angle, tile) in the (x, -y, z) frame: the eight tile
+// corners through the voxels' own float64 operations (each coordinate is
+// monotone in each output index) bound the coordinates of the tile.
+template <int ORDER>
+__global__ void k_affine_windows(const double *__restrict__ params, int nb, int ntx,
+                                 int nty, int ntz, int ny, int nz, int nx,
+                                 int4 *__restrict__ win) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nt = ntx * nty * ntz;
+  if (i >= (long long)nb * nt) return;
+  const int a = (int)(i / nt), ti = (int)(i - (long long)a * nt);
+  const int tx = ti % ntx, ty = (ti / ntx) % nty, tz = ti / (ntx * nty);
+  const double *p = params + 12 * (size_t)a;
+  const int x0 = tx * kAffTile, y0 = ty * kAffTile, z0 = tz * kAffTile;
+  const int x1 = min(x0 + kAffTile - 1, nx - 1), y1 = min(y0 + kAffTile - 1, ny - 1),
+            z1 = min(z0 + kAffTile - 1, nz - 1);
+  double lo[3], hi[3];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double o0 = (c & 1) ? x1 : x0;
+    const double o1 = (c & 2) ? (double)(ny - 1 - y0) : (double)(ny - 1 - y1);
+    const double o2 = (c & 4) ? z1 : z0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double v = affine3(o0, o1, o2, p[3 * d], p[3 * d + 1], p[3 * d + 2], p[9 + d]);
+      lo[d] = c ? fmin(lo[d], v) : v;
+      hi[d] = c ? fmax(hi[d], v) : v;
+    }
+  }
+  const int w0 = aff_axis<ORDER>(lo[0], hi[0], nx);
+  const int w1 = aff_axis<ORDER>(lo[1], hi[1], ny);
+  const int w2 = aff_axis<ORDER>(lo[2], hi[2], nz);
+  win[i] = make_int4(w0 & 0xffff, w1 & 0xffff, w2 & 0xffff,
+                     (w0 >> 16) | ((w1 >> 16) << 8) | ((w2 >> 16) << 16));
+}
+
+template <typename R, typename V, int ORDER>
+__global__ void __launch_bounds__(256)
+    k_affine_lds(const V *__restrict__ coef, int nb, const double *__restrict__ params,
+                 const int4 *__restrict__ win, V *__restrict__ out,
+                 V *__restrict__ comp, int ny, int nz, int nx, int first, int last,
+                 int ntx, int nty, int ntz, int xcd_map) {
+  constexpr int W = aff_win<ORDER>();
+  constexpr int NS = (W * W * W + 255) / 256;  // window slots per thread
+  __shared__ V buf[W * W * W];
+  const int nt = ntx * nty * ntz;
+  int ti = blockIdx.x;
+  if (xcd_map) {
+    // XCD x (= block id mod 8) takes tiles [x * nt / 8, (x + 1) * nt / 8)
+    const int per = nt / 8;
+    ti = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int tx = ti % ntx, ty = (ti / ntx) % nty, tz = ti / (ntx * nty);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = tx * kAffTile + (lane & 7), z = tz * kAffTile + (lane >> 3);
+  const int ya = ty * kAffTile + 2 * wave;
+  bool own[2];
+  size_t o[2];
+  wide_t<V> sum[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int y = ya + k;
+    own[k] = x < nx && z < nz && y < ny;
+    o[k] = ((size_t)z * ny + (own[k] ? y : 0)) * nx + x;
+    acc_zero(sum[k]);
+    if (!first && own[k]) sum[k] = load_sum(out, comp, o[k]);
+  }
+  const size_t vol = (size_t)ny * nz * nx;
+  // The window as a W x W x W box in LDS, slot (j, k, i) = sample
+  // (lo0 + i, lo1 + j, lo2 + k); a thread takes the slots t, t + 256, ...
+  // Per slot the part of the sample offset that does not depend on the
+  // window origin, -j*nz*nx + k*nx + i, and the packed (i, j, k) are formed
+  // once; per angle the thread issues all its loads before it stores any.
+  int soff[NS], spk[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int sl = (int)threadIdx.x + 256 * u;
+    const int j = sl / (W * W), k = (sl / W) % W, i = sl % W;
+    soff[u] = -j * nz * nx + k * nx + i;
+    spk[u] = sl < W * W * W ? i | (j << 8) | (k << 16) : 0x7f7f7f;
+  }
+  for (int a = 0; a < nb; ++a) {
+    const int4 e = win[(size_t)a * nt + ti];
+    const int lo0 = e.x, lo1 = e.y, lo2 = e.z;
+    const int n0 = e.w & 0xff, n1 = (e.w >> 8) & 0xff, n2 = (e.w >> 16) & 0xff;
+    const V *base = coef + (size_t)a * vol;
+    const V *org = base + ((size_t)(ny - 1 - lo1) * nz + lo2) * nx + lo0;
+    V vals[NS];
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int i = spk[u] & 0xff, j = (spk[u] >> 8) & 0xff, k = spk[u] >> 16;
+      const bool use = i < n0 && j < n1 && k < n2;
+      const V *src = use ? org + soff[u] : base;  // unused slots read sample 0
+      vals[u] = *src;
+    }
+    __syncthreads();  // the previous angle's gathers are done
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int sl = (int)threadIdx.x + 256 * u;
+      if (sl < W * W * W) buf[sl] = vals[u];
+    }
+    __syncthreads();
+    const double *p = params + 12 * a;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double o0 = (double)x, o1 = (double)(ny - 1 - (ya + k)), o2 = (double)z;
+      const double c0 = affine3(o0, o1, o2, p[0], p[1], p[2], p[9]);
+      const double c1 = affine3(o0, o1, o2, p[3], p[4], p[5], p[10]);
+      const double c2 = affine3(o0, o1, o2, p[6], p[7], p[8], p[11]);
+      const bool valid = own[k] && !(c0 < 0.0 || c0 > (double)(nx - 1) || c1 < 0.0 ||
+                                     c1 > (double)(ny - 1) || c2 < 0.0 ||
+                                     c2 > (double)(nz - 1));
+      if (!valid) continue;
+      R w0[ORDER + 1], w1[ORDER + 1], w2[ORDER + 1];
+      int i0[ORDER + 1], i1[ORDER + 1], i2[ORDER + 1];
+      spline_weights<ORDER, R>(c0, w0);
+      spline_weights<ORDER, R>(c1, w1);
+      spline_weights<ORDER, R>(c2, w2);
+      const int s0 = spline_start<ORDER>(c0), s1 = spline_start<ORDER>(c1),
+                s2 = spline_start<ORDER>(c2);
+#pragma unroll
+      for (int j = 0; j <= ORDER; ++j) {
+        i0[j] = min(max(mirror1(s0 + j, nx) - lo0, 0), W - 1);
+        i1[j] = min(max(mirror1(s1 + j, ny) - lo1, 0), W - 1) * W * W;
+        i2[j] = min(max(mirror1(s2 + j, nz) - lo2, 0), W - 1) * W;
+      }
+      V v;
+      acc_zero(v);
+#pragma unroll
+      for (int q = 0; q <= ORDER; ++q) {
+        V tq;
+        acc_zero(tq);
+#pragma unroll
+        for (int r = 0; r <= ORDER; ++r) {
+          const V *row = buf + i1[q] + i2[r];
+          V t;
+          acc_zero(t);
+#pragma unroll
+          for (int pp = 0; pp <= ORDER; ++pp) t = fma_w(w0[pp], row[i0[pp]], t);
+          tq = fma_w(w2[r], t, tq);
+        }
+        v = fma_w(w1[q], tq, v);
+      }
+      sum[k] = sum[k] + widen(v);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (own[k]) store_sum(out, comp, o[k], sum[k], !last);
+}
+
+// ---------------------------------------------------------------------------
 // In-place rotation with scipy's aliasing for spline orders 0 and 1.
 // The reference rotates each y-plane with
 // scipy.ndimage.rotate(arr[...], output=arr[...]) (_alg3d_bpp.py:73-81).
@@ -1968,10 +2161,37 @@ hipError_t launch_rotate_acc(int order, const V *coef, int nb,
   return hipErrorInvalidValue;
 }
 
+// ODTBRAIN_AMD_AFFINE_DIRECT: the tilted gather from global memory
+// (k_affine_acc) instead of the LDS-staged k_affine_lds.
+inline bool affine_direct() {
+  static const bool f = std::getenv("ODTBRAIN_AMD_AFFINE_DIRECT") != nullptr;
+  return f;
+}
+
 template <typename R, typename V, int ORDER>
 static hipError_t affine_dispatch(const V *coef, int nb, const double *params,
-                                  V *out, V *comp, int ny, int nz, int nx,
+                                  int *win, V *out, V *comp, int ny, int nz, int nx,
                                   int first, int last, hipStream_t st) {
+  // one reflection is the whole mirror rule when every edge is >= 4
+  // LDS staging for samples of at most 8 bytes (complex64, real types): the
+  // complex128 window and its registers leave too few blocks per CU
+  // (measured at 192^2 x 240: FP64 84 -> 191 ms per reconstruction)
+  if (sizeof(V) <= 8 && win && nx >= 4 && ny >= 4 && nz >= 4 && nb >= 1 &&
+      !affine_direct()) {
+    const int ntx = (nx + kAffTile - 1) / kAffTile, nty = (ny + kAffTile - 1) / kAffTile,
+              ntz = (nz + kAffTile - 1) / kAffTile;
+    const long long nt = (long long)ntx * nty * ntz;
+    const long long nw = nt * nb;
+    k_affine_windows<ORDER><<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(
+        params, nb, ntx, nty, ntz, ny, nz, nx, (int4 *)win);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int xcd_map = nt % 8 == 0 ? 1 : 0;
+    k_affine_lds<R, V, ORDER><<<(unsigned)nt, 256, 0, st>>>(
+        coef, nb, params, (const int4 *)win, out, comp, ny, nz, nx, first, last, ntx, nty,
+        ntz, xcd_map);
+    return hipGetLastError();
+  }
   dim3 block(256);
   dim3 grid((nx + 7) / 8, (ny + 3) / 4, (nz + 7) / 8);
   k_affine_acc<R, V, ORDER><<<grid, block, 0, st>>>(coef, nb, params, out, comp,
@@ -1981,16 +2201,16 @@ static hipError_t affine_dispatch(const V *coef, int nb, const double *params,
 
 template <typename R, typename V>
 hipError_t launch_affine_acc(int order, const V *coef, int nb,
-                             const double *params, V *out, V *comp, int ny,
+                             const double *params, int *win, V *out, V *comp, int ny,
                              int nz, int nx, int first, int last,
                              hipStream_t st) {
   switch (order) {
-    case 0: return affine_dispatch<R, V, 0>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 1: return affine_dispatch<R, V, 1>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 2: return affine_dispatch<R, V, 2>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 3: return affine_dispatch<R, V, 3>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 4: return affine_dispatch<R, V, 4>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
-    case 5: return affine_dispatch<R, V, 5>(coef, nb, params, out, comp, ny, nz, nx, first, last, st);
+    case 0: return affine_dispatch<R, V, 0>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 1: return affine_dispatch<R, V, 1>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 2: return affine_dispatch<R, V, 2>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 3: return affine_dispatch<R, V, 3>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 4: return affine_dispatch<R, V, 4>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
+    case 5: return affine_dispatch<R, V, 5>(coef, nb, params, win, out, comp, ny, nz, nx, first, last, st);
   }
   return hipErrorInvalidValue;
 }
@@ -2145,12 +2365,12 @@ template hipError_t launch_fwd_cols<float, double>(
       int, const cpx<R> *, int, const double *, int *, cpx<R> *, cpx<R> *,     \
       int, int, int, int, int, hipStream_t, int, cpx<R> *);                    \
   template hipError_t launch_affine_acc<R, R>(int, const R *, int,             \
-                                              const double *, R *, R *, int,   \
-                                              int, int, int, int,              \
+                                              const double *, int *, R *, R *, \
+                                              int, int, int, int, int,         \
                                               hipStream_t);                    \
   template hipError_t launch_affine_acc<R, cpx<R>>(                            \
-      int, const cpx<R> *, int, const double *, cpx<R> *, cpx<R> *, int, int,  \
-      int, int, int, hipStream_t);                                             \
+      int, const cpx<R> *, int, const double *, int *, cpx<R> *, cpx<R> *,     \
+      int, int, int, int, int, hipStream_t);                                   \
   template hipError_t launch_seq_rotate_sum<R, R>(int, R *, int,              \
                                                   const double *, R *, R *,    \
                                                   int, int, int, int, int,     \

@@ -67,9 +67,12 @@ hipError_t launch_rotate_acc(int order, const V *coef, int nb,
                              const double *params, int *win, V *out, V *comp,
                              int ny, int nz, int nx, int first, int last,
                              hipStream_t st, int groups = 1, V *part = nullptr);
+// Affine gather-accumulate of nb angles (tilted axis); win is scratch of
+// 4 * nb * affine_window_entries(ny, nz, nx) ints (null: the direct gather).
+size_t affine_window_entries(int ny, int nz, int nx);
 template <typename R, typename V>
 hipError_t launch_affine_acc(int order, const V *coef, int nb,
-                             const double *params, V *out, V *comp, int ny,
+                             const double *params, int *win, V *out, V *comp, int ny,
                              int nz, int nx, int first, int last,
                              hipStream_t st);
 template <typename R, typename V>

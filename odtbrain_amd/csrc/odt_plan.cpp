@@ -361,7 +361,8 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
         const int gn = std::min(p->RG, nb - g0);
         const size_t per_angle = (size_t)g.Ny * g.ln * g.Nx;
         HIPCHK((odt::launch_affine_acc<R, V>(p->order, slab + (size_t)g0 * per_angle, gn,
-                                             prm + (size_t)g0 * p->pstride, out, comp,
+                                             prm + (size_t)g0 * p->pstride,
+                                             (int *)(ws + p->off_win), out, comp,
                                              g.Ny, g.ln, g.Nx, first && g0 == 0,
                                              last && g0 + gn == nb, st)),
                "k_affine_acc");
@@ -558,6 +559,16 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
     return fail(ODT_ERR_INVALID, "angle_param_stride must be 6 (planar) or 12 (affine)");
   if (!d->weights || !d->angle_params)
     return fail(ODT_ERR_INVALID, "weights and angle_params are required");
+  if (d->kind == ODT_KIND_3D_TILTED) {
+    // the LDS windows of the affine gather are sized for rotations: a row of
+    // a rotation matrix has an L1 norm of at most sqrt(3)
+    for (int a = 0; a < d->A; ++a)
+      for (int r = 0; r < 3; ++r) {
+        const double *m = d->angle_params + 12 * (size_t)a + 3 * r;
+        if (!(std::fabs(m[0]) + std::fabs(m[1]) + std::fabs(m[2]) <= std::sqrt(3.0) + 1e-9))
+          return fail(ODT_ERR_INVALID, "angle_params: each 3x3 matrix must be a rotation");
+      }
+  }
   if (!(d->km > 0)) return fail(ODT_ERR_INVALID, "km must be positive");
   if (d->kind == ODT_KIND_2D && (d->Ny != 1 || d->lNy != 1))
     return fail(ODT_ERR_INVALID, "2D plans need Ny == lNy == 1");
@@ -666,10 +677,15 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
     p->off_comp = off;
     off = align256(off + slab_angle);
   }
-  // windows of the rotate-accumulate tiles (four ints per tile/angle)
+  // windows of the rotate-accumulate tiles (four ints per tile/angle): the
+  // planar tiles of a batch, or the 3D tiles of one tilted launch
   p->off_win = off;
-  off = align256(off + 4 * sizeof(int) * (size_t)p->NB *
-                           odt::rotate_window_entries(p->g.ln, p->g.Nx));
+  off = align256(off + 4 * sizeof(int) *
+                           (d->kind == ODT_KIND_3D_TILTED
+                                ? (size_t)p->RG * odt::affine_window_entries(
+                                                      p->g.Ny, p->g.ln, p->g.Nx)
+                                : (size_t)p->NB *
+                                      odt::rotate_window_entries(p->g.ln, p->g.Nx)));
   // 2D images have few tiles ((N/16)^2: 64 at N = 128) and each block loops
   // over every angle, so the angles run in concurrent groups (at least 8
   // angles each, about 2560 blocks in all, a sum and a compensation volume
