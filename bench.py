@@ -172,7 +172,7 @@ def c1_workload(cpu):
     return line
 
 
-def batch_bench(steps, warmup):
+def batch_bench(steps, warmup, lanes=4):
     """The batched workload (48 sinograms of uneven sizes) replayed as one
     HIP graph; aggregate voxel-angle updates/s (2D: N^2 per angle, 3D: N^3
     per angle), the eager sequential time beside it, and per-instance
@@ -190,8 +190,8 @@ def batch_bench(steps, warmup):
         jobs.append((fns[kind], (d, ang, res, nm), {"dtype": np.float32}))
         N = s.shape[-1]
         work += (N ** 2 if kind == "2d" else N ** 3) * len(ang)
-    log("bench: capturing the batch graph")
-    b = G.Batch(jobs)
+    log(f"bench: capturing the batch graph ({lanes} lanes)")
+    b = G.Batch(jobs, lanes=lanes)
     for _ in range(warmup):
         b.run()
     torch.cuda.synchronize()
@@ -220,6 +220,7 @@ def batch_bench(steps, warmup):
         "config": {"workload": "32 backpropagate_2d (N in 128..1024, "
                                "90-720 angles) + 16 backpropagate_3d "
                                "(N 64-160, 60-300 angles)"},
+        "lanes": lanes,
         "eager_sequential_ms": t_eager * 1e3,
         "fp32_rel_l2_vs_fp64_max": max(errs),
         "fp32_rel_l2_vs_fp64_per_instance": errs,
@@ -507,6 +508,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--batch", action="store_true",
                     help="run the batched 48-sinogram workload instead")
+    ap.add_argument("--batch-lanes", type=int, default=4,
+                    help="concurrent graph branches of --batch")
     ap.add_argument("--adjoint", action="store_true",
                     help="run the differentiable (gradient-descent) workload "
                          "instead; --steps defaults to 50 there")
@@ -534,7 +537,7 @@ def main():
         example_bench(args.example, args.steps, args.warmup)
         return
     if args.batch:
-        batch_bench(args.steps, args.warmup)
+        batch_bench(args.steps, args.warmup, args.batch_lanes)
         return
     if args.tilted:
         tilted_bench(args.steps, args.warmup)

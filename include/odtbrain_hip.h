@@ -162,6 +162,15 @@ typedef struct odt_batch odt_batch;
 int odt_batch_create(int n, odt_plan *const *plans, const void *const *d_usin,
                      void *const *d_out, void *d_ws, size_t ws_bytes,
                      void *stream, odt_batch **batch);
+/* The same with the plans spread over `lanes` concurrent branches of the
+ * graph (longest first onto the least loaded lane): d_ws holds `lanes`
+ * equal parts, each at least the largest odt_plan_workspace_size of the
+ * plans; the plans of one lane run in order on that lane's part.  Small
+ * reconstructions then overlap on the GPU instead of running one after
+ * another.  lanes = 1 is odt_batch_create. */
+int odt_batch_create_lanes(int n, odt_plan *const *plans, const void *const *d_usin,
+                           void *const *d_out, void *d_ws, size_t ws_bytes, int lanes,
+                           void *stream, odt_batch **batch);
 int odt_batch_launch(odt_batch *batch, void *stream);
 int odt_batch_destroy(odt_batch *batch);
 

@@ -21,7 +21,8 @@ EXPORTED = (
     "odt_plan_execute_noalloc", "odt_plan_destroy", "odt_plan_enable_timing", "odt_plan_last_timing",
     "odt_fft_c2c", "odt_spline_prefilter", "odt_rotate_accumulate",
     "odt_plan_adjoint",
-    "odt_batch_create", "odt_batch_launch", "odt_batch_destroy",
+    "odt_batch_create", "odt_batch_create_lanes", "odt_batch_launch",
+    "odt_batch_destroy",
     "odt_odt_to_ri", "odt_opt_to_ri",
     "odt_prepare_workspace_size", "odt_sinogram_prepare",
     "odt_apple_workspace_size", "odt_apple_core", "odt_apple_correct",
@@ -96,6 +97,8 @@ def lib():
     vpp = ctypes.POINTER(vp)
     L.odt_batch_create.argtypes = [ctypes.c_int, vpp, vpp, vpp, vp,
                                    ctypes.c_size_t, vp, vpp]
+    L.odt_batch_create_lanes.argtypes = [ctypes.c_int, vpp, vpp, vpp, vp,
+                                         ctypes.c_size_t, ctypes.c_int, vp, vpp]
     L.odt_batch_launch.argtypes = [vp, vp]
     L.odt_batch_destroy.argtypes = [vp]
     for name in ("odt_odt_to_ri", "odt_opt_to_ri"):

@@ -3,13 +3,15 @@
 The reference reconstructs one sinogram per call.  ``Batch`` records the
 calls of a list of jobs (any of backpropagate_3d, backpropagate_3d_tilted,
 backpropagate_2d with their usual arguments), builds one plan per job, and
-captures all executions on one stream into a HIP graph
-(``odt_batch_create``); ``run()`` replays the graph.  Outputs are GPU
-tensors owned by the batch and overwritten by every run.
+captures all executions into one HIP graph whose ``lanes`` branches run
+concurrently, each on its own part of the workspace (``odt_batch_create_lanes``;
+the jobs are spread over the lanes longest first); ``run()`` replays the
+graph.  Outputs are GPU tensors owned by the batch and overwritten by every
+run.
 
     jobs = [(odtbrain_amd.backpropagate_2d, (sino, angles, res, nm),
              {"dtype": np.float32}), ...]
-    b = Batch(jobs)
+    b = Batch(jobs)         # lanes=4 by default
     outs = b.run()          # list of torch tensors, one per job
 """
 import ctypes
@@ -19,7 +21,7 @@ from . import _native as N
 
 
 class Batch:
-    def __init__(self, jobs):
+    def __init__(self, jobs, lanes=4):
         torch = _core._torch()
         views = []
         with _core.collect() as col:
@@ -34,7 +36,10 @@ class Batch:
             if d_in.device != dev:
                 raise ValueError("all jobs of a batch must use one device")
         self.device = dev
-        ws = max(1, max(p.workspace_bytes for p, _, _ in self.entries))
+        self.lanes = max(1, min(int(lanes), len(self.entries)))
+        per = max(1, max(p.workspace_bytes for p, _, _ in self.entries))
+        per = (per + 255) // 256 * 256
+        ws = per * self.lanes
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
         # the null stream cannot be captured: a dedicated (default-priority)
         # stream runs the batch
@@ -47,11 +52,11 @@ class Batch:
         h = ctypes.c_void_p()
         cur = torch.cuda.current_stream(dev)
         self.stream.wait_stream(cur)
-        N.check(N.lib().odt_batch_create(
+        N.check(N.lib().odt_batch_create_lanes(
             n, self._plans, self._ins, self._outs,
-            ctypes.c_void_p(self.workspace.data_ptr()), ws,
+            ctypes.c_void_p(self.workspace.data_ptr()), ws, self.lanes,
             ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)),
-            "odt_batch_create")
+            "odt_batch_create_lanes")
         self.handle = h
 
     @property

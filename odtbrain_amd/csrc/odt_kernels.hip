@@ -1191,16 +1191,52 @@ struct RotTile {
   }
 };
 
+// s_waitcnt immediate (gfx9 encoding) waiting until no LDS (and scalar
+// memory) operation of this wave is outstanding; vector memory and export
+// counters untouched.
+__host__ __device__ constexpr int waitcnt_lgkm0() {
+  return 0xF | (0x7 << 4) | (0x0 << 8) | (0x3 << 14);
+}
+
 // Workgroup barrier without the memory fence of __syncthreads(): that fence
 // would also wait for the prefetch of the next tile.  Ordering is explicit
 // instead: each wave waits (vmcnt) for its own loads of the tile it is about
-// to read, and consumes its LDS reads of a tile before the barrier that
-// precedes the next prefetch into it.  The signal fences only stop the
-// compiler from moving memory operations across the barrier.
+// to read, and, before the barrier, for its LDS reads (lgkmcnt): a read
+// still in flight at the barrier could be serviced after another wave's
+// direct global->LDS load has refilled the buffer behind it (issuing the
+// reads before the barrier is not enough; the compiler consumes their
+// results later).  The signal fences stop the compiler from moving memory
+// operations across the barrier.
 __device__ __forceinline__ void barrier_no_fence() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(waitcnt_lgkm0());
   __builtin_amdgcn_s_barrier();
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Order in which the blocks of one y-chunk take the tiles: along a Z-order
+// curve for square power-of-two tile grids, else in 4 x 4 groups of tiles
+// (row-major groups) where the grid allows, else row-major.  Tiles whose
+// windows overlap then run close together in dispatch order, so a window
+// line one block fetched is still in the XCD's L2 when its vertical
+// neighbours ask for it (row-major order puts them a whole tile row apart).
+// Measured at 256^2 x 360, rotate stage per reconstruction: row-major 15.3
+// ms, 4 x 4 groups 14.6, Z-order 14.4 (48-angle launches).
+__device__ __forceinline__ int tile_order(int t, int ntx, int ntz) {
+  if (ntx == ntz && (ntx & (ntx - 1)) == 0) {
+    int tx = 0, tz = 0;
+    for (int b = 0; (1 << b) < ntx; ++b) {
+      tx |= ((t >> (2 * b)) & 1) << b;
+      tz |= ((t >> (2 * b + 1)) & 1) << b;
+    }
+    return tz * ntx + tx;
+  }
+  if (ntx % 4 == 0 && ntz % 4 == 0) {
+    const int g = t >> 4, w = t & 15;
+    const int gx = g % (ntx / 4), gz = g / (ntx / 4);
+    return (gz * 4 + (w >> 2)) * ntx + gx * 4 + (w & 3);
+  }
+  return t;
 }
 
 // ag > 0: angle groups along blockIdx.y; group gy takes angles
@@ -1236,7 +1272,7 @@ __global__ void __launch_bounds__(256)
   if (xcd_map) {
     const int bid = blockIdx.x;
     yc = (bid & 7) + 8 * (bid / (8 * nt));
-    ti = (bid >> 3) % nt;
+    ti = tile_order((bid >> 3) % nt, ntx, ntz);
     if (yc >= nyc) return;  // uniform per block
   } else {
     yc = blockIdx.x / nt;
@@ -1348,7 +1384,7 @@ __global__ void __launch_bounds__(256)
   if (xcd_map) {
     const int bid = blockIdx.x;
     yc = (bid & 7) + 8 * (bid / (8 * nt));
-    ti = (bid >> 3) % nt;
+    ti = tile_order((bid >> 3) % nt, ntx, ntz);
     if (yc >= nyc) return;  // uniform per block
   } else {
     yc = blockIdx.x / nt;

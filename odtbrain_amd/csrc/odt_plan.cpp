@@ -663,13 +663,13 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   // planes drift apart over many angles and then no longer share them in
   // L2, so long batches are split (the running sums go through the output
   // and its compensation term between launches).  Measured at 256^2 x 360
-  // (complex64, four-plane rotate blocks): 24 angles 15.2 ms per
-  // reconstruction, 32: 15.3, 48: 15.6, 64: 15.6; the tilted gather keeps
-  // the 48 it was measured with.
+  // (complex64, four-plane rotate blocks, Z-ordered tiles): 36 angles 14.6
+  // ms per reconstruction, 48: 14.4, 72: 14.5, 120: 14.9; the tilted gather
+  // keeps the 48 it was measured with.
   p->RG = p->NB;
   if ((d->kind == ODT_KIND_3D || d->kind == ODT_KIND_3D_TILTED) && d->intp_order >= 2) {
     const char *env = std::getenv("ODTBRAIN_AMD_ROTATE_GROUP");
-    const int rg = env ? std::atoi(env) : (d->kind == ODT_KIND_3D ? 24 : 48);
+    const int rg = env ? std::atoi(env) : 48;
     if (rg > 0) p->RG = std::min(p->NB, rg);
   }
   p->off_comp = 0;
@@ -818,50 +818,125 @@ struct odt_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int n = 0;
+  std::vector<hipStream_t> lanes;  // capture streams (default priority)
+  std::vector<hipEvent_t> events;  // fork and joins
 };
 
-int odt_batch_create(int n, odt_plan *const *plans, const void *const *d_usin,
-                     void *const *d_out, void *ws, size_t ws_bytes,
-                     void *stream, odt_batch **out) {
-  if (!out || n < 1 || !plans || !d_usin || !d_out || !stream)
+namespace {
+
+void free_batch(odt_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto s : b->lanes) (void)hipStreamDestroy(s);
+  for (auto e : b->events) (void)hipEventDestroy(e);
+  delete b;
+}
+
+// Estimated work of a plan (filter transforms plus resampling), for
+// balancing the lanes of a batch.
+double plan_cost(const odt_plan *p) {
+  const Geom &g = p->g;
+  return (double)p->A * g.ln * ((double)g.lNy * g.lNx + (double)g.Ny * g.Nx);
+}
+
+}  // namespace
+
+int odt_batch_create_lanes(int n, odt_plan *const *plans, const void *const *d_usin,
+                           void *const *d_out, void *ws, size_t ws_bytes, int lanes,
+                           void *stream, odt_batch **out) {
+  if (!out || n < 1 || !plans || !d_usin || !d_out || !stream || lanes < 1 || !ws)
     return fail(ODT_ERR_INVALID, "bad arguments (a non-null stream is required)");
   *out = nullptr;
+  lanes = std::min(lanes, n);
+  // each lane owns an equal, 256-byte aligned part of the workspace
+  const size_t part = (ws_bytes / (size_t)lanes) & ~(size_t)255;
   for (int i = 0; i < n; ++i) {
     if (!plans[i] || !d_usin[i] || !d_out[i])
       return fail(ODT_ERR_INVALID, "null plan or buffer in batch");
     if (plans[i]->timing) return fail(ODT_ERR_INVALID, "timing enabled in a batch plan");
-    if (ws_bytes < plans[i]->ws_bytes)
+    if (part < plans[i]->ws_bytes)
       return fail(ODT_ERR_WORKSPACE, "workspace too small for batch entry " +
-                                         std::to_string(i));
+                                         std::to_string(i) + " (need " +
+                                         std::to_string(plans[i]->ws_bytes) +
+                                         " bytes per lane)");
   }
+  // longest first onto the least loaded lane
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return plan_cost(plans[a]) > plan_cost(plans[b]);
+  });
+  std::vector<int> lane_of(n);
+  std::vector<double> load(lanes, 0.0);
+  for (int i : order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lane_of[i] = l;
+    load[l] += plan_cost(plans[i]);
+  }
+  auto lane_ws = [&](int l) { return (void *)((char *)ws + (size_t)l * part); };
   hipStream_t st = (hipStream_t)stream;
   // one eager pass: validates every entry and sets kernel attributes
   // outside the capture
   for (int i = 0; i < n; ++i) {
-    const int rc = odt_plan_execute(plans[i], d_usin[i], d_out[i], ws, ws_bytes, stream);
+    const int rc = odt_plan_execute(plans[i], d_usin[i], d_out[i], lane_ws(lane_of[i]),
+                                    part, stream);
     if (rc != ODT_OK) return rc;
   }
   HIPCHK(hipStreamSynchronize(st), "batch warm-up");
   odt_batch *b = new odt_batch;
   b->n = n;
-  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+  hipError_t e = hipSuccess;
+  for (int l = 0; l < lanes && e == hipSuccess; ++l) {
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) b->lanes.push_back(s);
+  }
+  for (int k = 0; k < lanes + 1 && e == hipSuccess; ++k) {
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) b->events.push_back(ev);
+  }
+  if (e != hipSuccess) {
+    free_batch(b);
+    return hip_fail(e, "batch streams");
+  }
   int rc = ODT_OK;
+  e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
   if (e == hipSuccess) {
-    for (int i = 0; i < n && rc == ODT_OK; ++i)
-      rc = odt_plan_execute(plans[i], d_usin[i], d_out[i], ws, ws_bytes, stream);
+    hipError_t ec = hipEventRecord(b->events[0], st);  // fork
+    for (int l = 0; l < lanes && ec == hipSuccess; ++l)
+      ec = hipStreamWaitEvent(b->lanes[l], b->events[0], 0);
+    for (int i : order) {
+      if (ec != hipSuccess || rc != ODT_OK) break;
+      const int l = lane_of[i];
+      rc = odt_plan_execute(plans[i], d_usin[i], d_out[i], lane_ws(l), part, b->lanes[l]);
+    }
+    for (int l = 0; l < lanes && ec == hipSuccess; ++l) {  // joins
+      ec = hipEventRecord(b->events[l + 1], b->lanes[l]);
+      if (ec == hipSuccess) ec = hipStreamWaitEvent(st, b->events[l + 1], 0);
+    }
     const hipError_t e2 = hipStreamEndCapture(st, &b->graph);
-    if (e == hipSuccess) e = e2;
+    e = ec != hipSuccess ? ec : e2;
   }
   if (e == hipSuccess && rc == ODT_OK)
     e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
   if (e != hipSuccess || rc != ODT_OK) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    delete b;
-    return rc != ODT_OK ? rc : hip_fail(e, "batch capture");
+    const std::string keep = g_err;
+    free_batch(b);
+    if (rc != ODT_OK) {
+      g_err = keep;
+      return rc;
+    }
+    return hip_fail(e, "batch capture");
   }
   *out = b;
   return ODT_OK;
+}
+
+int odt_batch_create(int n, odt_plan *const *plans, const void *const *d_usin,
+                     void *const *d_out, void *ws, size_t ws_bytes,
+                     void *stream, odt_batch **out) {
+  return odt_batch_create_lanes(n, plans, d_usin, d_out, ws, ws_bytes, 1, stream, out);
 }
 
 int odt_batch_launch(odt_batch *b, void *stream) {
@@ -871,10 +946,7 @@ int odt_batch_launch(odt_batch *b, void *stream) {
 }
 
 int odt_batch_destroy(odt_batch *b) {
-  if (!b) return ODT_OK;
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  delete b;
+  if (b) free_batch(b);
   return ODT_OK;
 }
 

@@ -29,10 +29,13 @@ def _jobs():
     ]
 
 
-def test_batch_graph_replays_equal_eager_and_oracle():
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_batch_graph_replays_equal_eager_and_oracle(lanes):
+    """One lane (the plans in order on one stream) and three concurrent
+    lanes (each with its own part of the workspace)."""
     jobs = _jobs()
     eager = [fn(*a, **kw) for fn, a, kw in jobs]
-    b = G.Batch(jobs)
+    b = G.Batch(jobs, lanes=lanes)
     other_s, other_a = phantoms.disk_sinogram_2d(N=64, A=20, seed=9)
     for rep in range(3):
         outs = b.run()
