@@ -103,6 +103,28 @@ typedef struct odt_plan odt_plan;
  * the tables only. */
 int odt_plan_create(const odt_desc *desc, odt_plan **plan);
 
+/* Launch-shape choices of a plan that do not change what it computes: the
+ * reference has none (it has no device code); they are what the autotuner
+ * (odtbrain_amd.tune) searches.  0 in a field: the planner's default.
+ * Explicit values take precedence over the ODTBRAIN_AMD_* environment
+ * switches.  t_budget_bytes only re-chunks the same kernels' arithmetic
+ * (bit-identical results); rotate_group regroups the float32 running sums
+ * between launches (float64 plans stay bit-identical); rotate_groups_2d
+ * regroups the sums over angles and col_per_angle selects another kernel
+ * for the same values (both: rounding-level differences). */
+typedef struct odt_tuning {
+  int32_t rotate_group;     /* angles per rotate-accumulate launch (3D)   */
+  int32_t rotate_groups_2d; /* concurrent angle groups of the 2D rotate    */
+  int64_t t_budget_bytes;   /* budget of the intermediate transform chunk  */
+  int32_t col_per_angle;    /* 1: column stage with one block per angle,
+                               2: propagator held across a chunk's angles  */
+  int32_t reserved;
+} odt_tuning;
+
+/* odt_plan_create with launch-shape choices (tuning may be null). */
+int odt_plan_create_tuned(const odt_desc *desc, const odt_tuning *tuning,
+                          odt_plan **plan);
+
 /* Bytes of caller-provided device workspace that odt_plan_execute needs. */
 int odt_plan_workspace_size(const odt_plan *plan, size_t *bytes);
 

@@ -60,7 +60,7 @@ def _key_array(a):
 class Plan:
     """Owns one odt_plan (device tables) for a fixed description."""
 
-    def __init__(self, desc_fields, weights, params, device):
+    def __init__(self, desc_fields, weights, params, device, tuning=None):
         self.weights = np.ascontiguousarray(weights, dtype=np.float64)
         self.params = np.ascontiguousarray(params, dtype=np.float64)
         d = N.OdtDesc()
@@ -69,10 +69,18 @@ class Plan:
         d.weights = N.dptr(self.weights)
         d.angle_params = N.dptr(self.params)
         self.desc = d
+        self.desc_fields = dict(desc_fields)
         self.device = device
+        self.tuning = dict(tuning) if tuning else None
+        t = None
+        if self.tuning:
+            t = N.OdtTuning()
+            for k, v in self.tuning.items():
+                setattr(t, k, int(v))
         h = ctypes.c_void_p()
-        N.check(N.lib().odt_plan_create(ctypes.byref(d), ctypes.byref(h)),
-                "odt_plan_create")
+        N.check(N.lib().odt_plan_create_tuned(
+            ctypes.byref(d), ctypes.byref(t) if t is not None else None,
+            ctypes.byref(h)), "odt_plan_create_tuned")
         self.handle = h
         size = ctypes.c_size_t()
         N.check(N.lib().odt_plan_workspace_size(h, ctypes.byref(size)),
@@ -160,13 +168,16 @@ def slab_budget(per_angle_bytes, A, device):
 
 
 def get_plan(desc_fields, weights, params, device):
+    from . import tune
+    tuning = tune.lookup(desc_fields)
     key = (tuple(sorted(desc_fields.items())), _key_array(weights),
-           _key_array(params), str(device))
+           _key_array(params), str(device),
+           tuple(sorted(tuning.items())) if tuning else None)
     plan = _PLAN_CACHE.get(key)
     if plan is not None:
         _PLAN_CACHE.move_to_end(key)
         return plan
-    plan = Plan(desc_fields, weights, params, device)
+    plan = Plan(desc_fields, weights, params, device, tuning=tuning)
     _PLAN_CACHE[key] = plan
     while len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
         _PLAN_CACHE.popitem(last=False)
@@ -224,7 +235,9 @@ def run(desc_fields, weights, params, d_usin, out_shape, out_real,
     else:
         odt = torch.float32 if out_real else torch.complex64
     if _collector is not None:
-        plan = Plan(desc_fields, weights, params, dev)  # owned by the batch
+        from . import tune
+        plan = Plan(desc_fields, weights, params, dev,
+                    tuning=tune.lookup(desc_fields))  # owned by the batch
         out = torch.empty(out_shape, dtype=odt, device=dev)
         _collector.entries.append((plan, d_usin, out))
         return out  # filled by every replay of the batch

@@ -17,7 +17,8 @@ ODT_FLAG_IIR_PREFILTER = 1
 ODT_SCRATCH_BYTES = 128 << 20  # static workspace of odt_plan_execute_noalloc
 
 EXPORTED = (
-    "odt_plan_create", "odt_plan_workspace_size", "odt_plan_execute",
+    "odt_plan_create", "odt_plan_create_tuned", "odt_plan_workspace_size",
+    "odt_plan_execute",
     "odt_plan_execute_noalloc", "odt_plan_destroy", "odt_plan_enable_timing", "odt_plan_last_timing",
     "odt_fft_c2c", "odt_spline_prefilter", "odt_rotate_accumulate",
     "odt_plan_adjoint",
@@ -61,6 +62,16 @@ class OdtDesc(ctypes.Structure):
     ]
 
 
+class OdtTuning(ctypes.Structure):
+    """odt_tuning (include/odtbrain_hip.h): launch-shape choices, 0 = the
+    planner's default."""
+    _fields_ = [("rotate_group", ctypes.c_int32),
+                ("rotate_groups_2d", ctypes.c_int32),
+                ("t_budget_bytes", ctypes.c_int64),
+                ("col_per_angle", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class OdtError(RuntimeError):
     """Error reported by libodtbrain_hip."""
 
@@ -80,6 +91,9 @@ def lib():
     vp, i32, i64, dp = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
                         ctypes.POINTER(ctypes.c_double))
     L.odt_plan_create.argtypes = [ctypes.POINTER(OdtDesc), ctypes.POINTER(vp)]
+    L.odt_plan_create_tuned.argtypes = [ctypes.POINTER(OdtDesc),
+                                        ctypes.POINTER(OdtTuning),
+                                        ctypes.POINTER(vp)]
     L.odt_plan_workspace_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.odt_plan_execute.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp]
     L.odt_plan_execute_noalloc.argtypes = [vp, vp, vp, vp]

@@ -288,13 +288,12 @@ __global__ void __launch_bounds__(256)
 // from evicting the L2 lines that the writing kernel still reads (measured
 // at 256^2 x 360: 58.1 -> 54.6 ms per reconstruction).
 typedef float f2v __attribute__((ext_vector_type(2)));
-typedef double d2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void st_nt(cpx<float> *p, cpx<float> v) {
   f2v x = {v.re, v.im}; __builtin_nontemporal_store(x, reinterpret_cast<f2v *>(p));
 }
-__device__ __forceinline__ void st_nt(cpx<double> *p, cpx<double> v) {
-  d2v x = {v.re, v.im}; __builtin_nontemporal_store(x, reinterpret_cast<d2v *>(p));
-}
+// 16-byte non-temporal stores measured slow (256^2 x 360 in FP64: 155 ->
+// 346 ms per reconstruction): complex128 samples take plain stores.
+__device__ __forceinline__ void st_nt(cpx<double> *p, cpx<double> v) { *p = v; }
 __device__ __forceinline__ void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
 
@@ -1937,7 +1936,8 @@ static hipError_t col_fft8_launch(const cpx<R> *S, cpx<R> *T,
   const bool band = in_band(g.jy0, g.Ly, N);
   // propagator in registers across the chunk's angles (see k_col_fft8_ang)
   if constexpr (TT <= 64) {
-    if (band && gtab && staged && !col_per_angle()) {
+    const bool per_angle = g.col_mode == 1 || (g.col_mode == 0 && col_per_angle());
+    if (band && gtab && staged && !per_angle) {
       constexpr int TPBa = 256 / TT < kColCpb ? 256 / TT : kColCpb;
       const size_t sha =
           ((size_t)TPBa * lds_padded(N) + (size_t)g.Ny * (kColCpb + 1)) * sizeof(cpx<R>);

@@ -18,6 +18,7 @@ struct Geom {
   int Lx, Ly;            // support box (x- and y-frequencies kept)
   int jx0, jy0;          // signed frequency index of support column/row 0
   int fuse_x, fuse_y, fuse_z;  // prefilter folded into the transforms
+  int col_mode;          // column stage: 0 default, 1 per angle, 2 angle loop
   double b1;             // B-spline sample at +-1 (1/8 order 2, 1/6 order 3)
   double z0;             // z of slice 0
 };

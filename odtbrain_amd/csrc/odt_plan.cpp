@@ -538,7 +538,14 @@ const char *odt_last_error(void) { return g_err.c_str(); }
 int odt_version(void) { return 1; }
 
 int odt_plan_create(const odt_desc *d, odt_plan **out) {
+  return odt_plan_create_tuned(d, nullptr, out);
+}
+
+int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **out) {
   if (!d || !out) return fail(ODT_ERR_INVALID, "null argument");
+  if (tu && (tu->rotate_group < 0 || tu->rotate_groups_2d < 0 || tu->t_budget_bytes < 0 ||
+             tu->col_per_angle < 0 || tu->col_per_angle > 2))
+    return fail(ODT_ERR_INVALID, "tuning: negative or unknown values");
   *out = nullptr;
   if (d->kind < 0 || d->kind > 2) return fail(ODT_ERR_INVALID, "unknown kind");
   if (d->precision != ODT_FP32 && d->precision != ODT_FP64)
@@ -614,6 +621,7 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   g.fuse_x = fused && g.Nx > 1;
   g.fuse_z = fused && g.ln > 1;
   g.fuse_y = fused && d->kind == ODT_KIND_3D_TILTED && g.Ny > 1;
+  g.col_mode = tu ? tu->col_per_angle : 0;
 
   p->esz_c = esz;
   p->esz_v = p->onlyreal ? esz / 2 : esz;
@@ -638,6 +646,7 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
       t_budget = std::min(t_budget, std::max(total_b / 32, (size_t)1 << 30));
   }
   if (tenv) t_budget = (size_t)std::atoll(tenv);
+  if (tu && tu->t_budget_bytes > 0) t_budget = (size_t)tu->t_budget_bytes;
   p->NC = (int)std::max<size_t>(1, std::min<size_t>(t_budget / t_angle, (size_t)p->NB));
   p->NC = std::min(p->NC, 64);
   // float64 forward stage where its transforms fit the LDS in float64
@@ -669,7 +678,7 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
   p->RG = p->NB;
   if ((d->kind == ODT_KIND_3D || d->kind == ODT_KIND_3D_TILTED) && d->intp_order >= 2) {
     const char *env = std::getenv("ODTBRAIN_AMD_ROTATE_GROUP");
-    const int rg = env ? std::atoi(env) : 48;
+    const int rg = tu && tu->rotate_group > 0 ? tu->rotate_group : env ? std::atoi(env) : 48;
     if (rg > 0) p->RG = std::min(p->NB, rg);
   }
   p->off_comp = 0;
@@ -698,6 +707,7 @@ int odt_plan_create(const odt_desc *d, odt_plan **out) {
     gg = std::min(gg, std::max(1, p->NB / 8));
     if (const char *genv = std::getenv("ODTBRAIN_AMD_ROTATE_2D_GROUPS"))
       gg = std::max(1, std::atoi(genv));
+    if (tu && tu->rotate_groups_2d > 0) gg = tu->rotate_groups_2d;
     if (gg > 1) {
       p->RGG = gg;
       p->off_part = off;

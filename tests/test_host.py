@@ -1,6 +1,7 @@
 """Host logic of odtbrain_amd and the C ABI library: CPU only (no compute
 call reaches the GPU)."""
 import ctypes
+import json
 import multiprocessing as mp
 import re
 
@@ -182,3 +183,34 @@ def test_rotate_accumulate_rejects_non_rotation():
                                        N.dptr(params), dummy, 0, None)
     assert rc == N.ODT_ERR_INVALID
     assert b"rotation" in N.lib().odt_last_error()
+
+
+def test_tuning_cache_format_and_lookup(tmp_path, monkeypatch):
+    """The autotuner's cache: entries round-trip through the JSON format,
+    the user cache overrides, files of another format are ignored, and
+    ODTBRAIN_AMD_NO_TUNING disables lookups."""
+    from odtbrain_amd import tune
+    path = tmp_path / "t.json"
+    monkeypatch.setenv("ODTBRAIN_AMD_TUNE_CACHE", str(path))
+    tune.reload()
+    desc = dict(kind=0, precision=0, A=40, Ny=48, Nx=48, lNy=128, lNx=128,
+                intp_order=2, onlyreal=0, flags=0)
+    assert tune.lookup(desc) is None
+    tune.save(tune.shape_key(desc), {"rotate_group": 36, "unknown": 5})
+    data = json.loads(path.read_text())
+    assert data["format"] == "odtbrain_amd-tuning" and data["version"] == 1
+    assert tune.lookup(desc) == {"rotate_group": 36}
+    assert tune.lookup(dict(desc, A=41)) is None
+    monkeypatch.setenv("ODTBRAIN_AMD_NO_TUNING", "1")
+    assert tune.lookup(desc) is None
+    monkeypatch.delenv("ODTBRAIN_AMD_NO_TUNING")
+    path.write_text(json.dumps({"format": "other", "entries": {}}))
+    tune.reload()
+    assert tune.lookup(desc) is None
+    tune.reload()
+
+
+def test_tuning_struct_layout():
+    from odtbrain_amd import _native as N
+    assert ctypes.sizeof(N.OdtTuning) == 24
+    assert N.OdtTuning.t_budget_bytes.offset == 8
