@@ -198,7 +198,10 @@ int odt_batch_destroy(odt_batch *batch);
 
 /* Batched 1D complex transforms along the last axis, unnormalised.
  * direction -1: forward (exp(-2 pi i jk/n)), +1: backward.  n arbitrary
- * (power of two: Stockham; otherwise direct DFT).  In place allowed. */
+ * (power of two: Stockham; other lengths up to 2048: Bluestein's chirp
+ * convolution on power-of-two transforms; longer: direct DFT), as the
+ * reference's FFTW plans take any length (_alg3d_bpp.py:460-478).  In place
+ * allowed. */
 int odt_fft_c2c(int precision, int64_t batch, int32_t n, int32_t direction,
                 const void *d_in, void *d_out, void *stream);
 

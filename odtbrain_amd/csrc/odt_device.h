@@ -1,8 +1,8 @@
 // Device building blocks for the gfx950 filtered-backpropagation kernels:
 // complex arithmetic, workgroup-cooperative LDS transforms (Stockham radix-2
-// for powers of two, direct DFT otherwise), B-spline tap weights and the
-// mirror index rule of scipy.ndimage (the resampler the reference calls in
-// odtbrain/_alg3d_bpp.py:73-81 and _alg3d_bppt.py:931-944).
+// for powers of two, Bluestein's chirp convolution otherwise), B-spline tap
+// weights and the mirror index rule of scipy.ndimage (the resampler the
+// reference calls in odtbrain/_alg3d_bpp.py:73-81 and _alg3d_bppt.py:931-944).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,8 +45,9 @@ __host__ __device__ __forceinline__ cpx<R> conj_if(cpx<R> a, bool c) {
 
 // ---------------------------------------------------------------------------
 // LDS transforms.  A workgroup transforms `nb` vectors of length n stored
-// contiguously in LDS buffer x (scratch y of the same size).  tw holds
-// exp(+2 pi i m / n), m = 0..n-1; dir < 0 uses the conjugate (forward).
+// contiguously in LDS buffer x (x and the scratch y each hold nb * lds_len(n)
+// entries).  For powers of two tw holds exp(+2 pi i m / n), m = 0..n-1
+// (other lengths: see lds_bluestein); dir < 0 uses the conjugate (forward).
 // Returns the buffer holding the result.  Both end with __syncthreads().
 // ---------------------------------------------------------------------------
 
@@ -81,7 +82,68 @@ __device__ cpx<R> *lds_fft_pow2(cpx<R> *x, cpx<R> *y, int n, int logn, int nb,
   return x;
 }
 
-// Direct DFT for lengths that are not powers of two (no-padding case).
+// Lengths that are not powers of two (the no-padding case, where the
+// reference's FFTW plans take any size) use Bluestein's algorithm up to
+// kBluesteinMax: with c_j = exp(s i pi j^2 / n),
+//   X_k = sum_j x_j exp(s 2 pi i jk / n) = c_k sum_j (x_j c_j) conj(c_{k-j}),
+// a linear convolution evaluated circularly with power-of-two transforms of
+// length m >= 2n - 1.  Longer ones take the direct DFT.  The twiddle table
+// of such a length holds 2m + n entries: exp(2 pi i k / m) (k < m), the
+// chirp w_j = exp(i pi j^2 / n) (j < n), and H = FFT_m(h) / m of the
+// kernel h_d = conj(w_|d|) placed at d and m - d (the s = +1 kernel; the
+// s = -1 one is its conjugate, and H is even, so its transform is conj(H)).
+constexpr int kBluesteinMax = 2048;
+
+__host__ __device__ __forceinline__ bool is_pow2(int n) {
+  return n > 0 && (n & (n - 1)) == 0;
+}
+__host__ __device__ __forceinline__ int bluestein_len(int n) {
+  int m = 1;
+  while (m < 2 * n - 1) m <<= 1;
+  return m;
+}
+__host__ __device__ __forceinline__ bool uses_bluestein(int n) {
+  return !is_pow2(n) && n <= kBluesteinMax;
+}
+// LDS entries per vector that the transform of length n works in
+__host__ __device__ __forceinline__ int lds_len(int n) {
+  return uses_bluestein(n) ? bluestein_len(n) : n;
+}
+// entries of the twiddle table of length n
+__host__ __device__ __forceinline__ int dft_table_len(int n) {
+  return uses_bluestein(n) ? 2 * bluestein_len(n) + n : n;
+}
+
+// Bluestein transform of nb vectors of length n (stride n in x); x and y
+// each hold nb * bluestein_len(n) entries.  Returns the buffer with the
+// result at stride n.
+template <typename R>
+__device__ cpx<R> *lds_bluestein(cpx<R> *x, cpx<R> *y, int n, int nb,
+                                 const cpx<R> *__restrict__ tw, int dir) {
+  const int m = bluestein_len(n);
+  const int logm = 31 - __clz(m);
+  const cpx<R> *chirp = tw + m, *H = tw + m + n;
+  const bool fwd = dir < 0;
+  for (int idx = threadIdx.x; idx < nb * m; idx += blockDim.x) {
+    const int b = idx >> logm, j = idx & (m - 1);
+    y[idx] = j < n ? x[b * n + j] * conj_if(chirp[j], fwd) : mk<R>(0, 0);
+  }
+  __syncthreads();
+  cpx<R> *z = lds_fft_pow2<R>(y, x, m, logm, nb, tw, -1);
+  for (int idx = threadIdx.x; idx < nb * m; idx += blockDim.x)
+    z[idx] = z[idx] * conj_if(H[idx & (m - 1)], fwd);
+  __syncthreads();
+  z = lds_fft_pow2<R>(z, z == y ? x : y, m, logm, nb, tw, +1);
+  cpx<R> *o = z == y ? x : y;
+  for (int idx = threadIdx.x; idx < nb * n; idx += blockDim.x) {
+    const int b = idx / n, k = idx - b * n;
+    o[idx] = z[b * m + k] * conj_if(chirp[k], fwd);
+  }
+  __syncthreads();
+  return o;
+}
+
+// Direct DFT for the lengths beyond kBluesteinMax that are not powers of two.
 template <typename R>
 __device__ cpx<R> *lds_dft(cpx<R> *x, cpx<R> *y, int n, int nb,
                            const cpx<R> *__restrict__ tw, int dir) {
@@ -106,6 +168,7 @@ __device__ __forceinline__ cpx<R> *lds_transform(cpx<R> *x, cpx<R> *y, int n,
                                                  int logn, int nb,
                                                  const cpx<R> *tw, int dir) {
   if (logn >= 0) return lds_fft_pow2<R>(x, y, n, logn, nb, tw, dir);
+  if (uses_bluestein(n)) return lds_bluestein<R>(x, y, n, nb, tw, dir);
   return lds_dft<R>(x, y, n, nb, tw, dir);
 }
 

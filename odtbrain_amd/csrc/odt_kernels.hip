@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256)
                const cpx<C> *__restrict__ twx, Geom g, double pad_re,
                double pad_im, int a0, int nb) {
   cpx<C> *buf = reinterpret_cast<cpx<C> *>(odt_smem);
-  cpx<C> *tmp = buf + nb * g.lNx;
+  cpx<C> *tmp = buf + nb * lds_len(g.lNx);
   const int ai = blockIdx.y;
   const int a = a0 + ai;
   // edge padding: the padded rows outside the detector repeat its first or
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256)
                const cpx<C> *__restrict__ twy, const cpx<C> *__restrict__ P,
                Geom g, int nb) {
   cpx<C> *buf = reinterpret_cast<cpx<C> *>(odt_smem);
-  cpx<C> *tmp = buf + nb * g.lNy;
+  cpx<C> *tmp = buf + nb * lds_len(g.lNy);
   const int ai = blockIdx.y;
   const int c0 = blockIdx.x * nb;
   const bool edge = g.pad_mode == 1;
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(256)
                const R *__restrict__ q0z, const R *__restrict__ q0y, Geom g,
                int nb, int ldt, const cpx<R> *__restrict__ gtab) {
   cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
-  cpx<R> *tmp = buf + nb * g.lNy;
+  cpx<R> *tmp = buf + nb * lds_len(g.lNy);
   __shared__ cpx<R> corr[2 * 16];
   const int c = blockIdx.x;
   const int k0 = blockIdx.y * nb;
@@ -752,7 +752,7 @@ __global__ void __launch_bounds__(256)
                const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
                Geom g, int nb, int slab0, int ldt) {
   cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
-  cpx<R> *tmp = buf + nb * g.lNx;
+  cpx<R> *tmp = buf + nb * lds_len(g.lNx);
   __shared__ cpx<R> corr[2 * 16];
   const int y0 = blockIdx.x * nb;
   const int k = blockIdx.y;
@@ -1820,7 +1820,7 @@ __global__ void __launch_bounds__(256)
     k_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch, int n,
                 int logn, int dir, const cpx<R> *tw, int nb) {
   cpx<R> *buf = reinterpret_cast<cpx<R> *>(odt_smem);
-  cpx<R> *tmp = buf + nb * n;
+  cpx<R> *tmp = buf + nb * lds_len(n);
   const long long b0 = (long long)blockIdx.x * nb;
   for (int idx = threadIdx.x; idx < nb * n; idx += blockDim.x) {
     const long long b = b0 + idx / n;
@@ -1861,7 +1861,7 @@ __global__ void __launch_bounds__(512)
 // launchers
 // ---------------------------------------------------------------------------
 static size_t lds_bytes(int n, int nb, size_t esz) {
-  return (size_t)2 * n * nb * esz;
+  return (size_t)2 * lds_len(n) * nb * esz;
 }
 
 static hipError_t set_lds(const void *fn, size_t bytes) {
@@ -1874,6 +1874,64 @@ template <typename R>
 hipError_t launch_twiddles(cpx<R> *tw, int n, hipStream_t st) {
   k_twiddles<R><<<(n + 255) / 256, 256, 0, st>>>(tw, n);
   return hipGetLastError();
+}
+
+// Bluestein tables of length n (m = bluestein_len(n), tw already holding
+// the m twiddles): the chirp w_j = exp(i pi j^2 / n) at tw + m, with j^2
+// reduced mod 2n exactly, and the kernel h_d = conj(w_d) / m at positions d
+// and m - d (zero between) at tw + m + n, transformed afterwards.
+__device__ __forceinline__ cpx<double> chirp_of(long long j, int n) {
+  double s, c;
+  sincospi((double)((j * j) % (2LL * n)) / (double)n, &s, &c);
+  return mk<double>(c, s);
+}
+
+__global__ void k_bluestein_chirp(cpx<double> *tw, int n, int m) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  if (j < n) tw[m + j] = chirp_of(j, n);
+  const int d = j < n ? j : (j > m - n ? m - j : -1);
+  cpx<double> h = mk<double>(0, 0);
+  if (d >= 0) {
+    const cpx<double> w = chirp_of(d, n);
+    h = mk<double>(w.re / m, -w.im / m);
+  }
+  tw[m + n + j] = h;
+}
+
+template <typename R>
+__global__ void k_narrow_table(const cpx<double> *__restrict__ in, cpx<R> *__restrict__ out,
+                               int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = mk<R>((R)in[i].re, (R)in[i].im);
+}
+
+template <typename R>
+hipError_t launch_fft_batch(const cpx<R> *in, cpx<R> *out, long long batch, int n,
+                            int logn, int dir, const cpx<R> *tw, int nb,
+                            hipStream_t st);
+
+// Twiddle table of a transform of length n (dft_table_len(n) entries).
+// Bluestein tables are built in float64 (in tw itself for float64 tables,
+// else in scratch64, dft_table_len(n) entries) and rounded once.
+template <typename R>
+hipError_t launch_dft_table(cpx<R> *tw, int n, cpx<double> *scratch64, hipStream_t st) {
+  if (!uses_bluestein(n)) return launch_twiddles<R>(tw, n, st);
+  const int m = bluestein_len(n), len = dft_table_len(n);
+  cpx<double> *s = sizeof(R) == sizeof(double) ? (cpx<double> *)tw : scratch64;
+  if (!s) return hipErrorInvalidValue;
+  hipError_t e = launch_twiddles<double>(s, m, st);
+  if (e != hipSuccess) return e;
+  k_bluestein_chirp<<<(m + 255) / 256, 256, 0, st>>>(s, n, m);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = launch_fft_batch<double>(s + m + n, s + m + n, 1, m, 31 - __builtin_clz(m), -1, s, 1,
+                               st);
+  if (e != hipSuccess) return e;
+  if (sizeof(R) != sizeof(double)) {
+    k_narrow_table<R><<<(len + 255) / 256, 256, 0, st>>>(s, tw, len);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 template <typename RI, typename C>
@@ -2371,6 +2429,8 @@ template hipError_t launch_fwd_cols<float, double>(
     const Geom &, int, int, hipStream_t);
 #define ODT_INST(R)                                                            \
   template hipError_t launch_twiddles<R>(cpx<R> *, int, hipStream_t);          \
+  template hipError_t launch_dft_table<R>(cpx<R> *, int, cpx<double> *,       \
+                                          hipStream_t);                      \
   template hipError_t launch_fwd_rows<R, R>(const cpx<R> *, const double *,    \
                                             cpx<R> *, const cpx<R> *,          \
                                             const Geom &, double, double, int, \

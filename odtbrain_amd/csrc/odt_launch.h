@@ -33,6 +33,11 @@ struct PropAux {
 
 template <typename R>
 hipError_t launch_twiddles(cpx<R> *tw, int n, hipStream_t st);
+// Twiddle table of a transform of length n, dft_table_len(n) entries (the
+// Bluestein tables for lengths that are not powers of two; float32 tables
+// are built in scratch64, dft_table_len(n) float64 entries, and rounded).
+template <typename R>
+hipError_t launch_dft_table(cpx<R> *tw, int n, cpx<double> *scratch64, hipStream_t st);
 // Forward stage: pad, transforms and prefactor computed in C (float64 for
 // both precisions where the transform fits the LDS in float64), the support
 // spectrum S stored in R.

@@ -53,6 +53,17 @@ int ilog2_exact(int n) {
   return l;
 }
 
+// Device allocation released at scope exit (hipFree waits for the device).
+struct DevBuf {
+  void *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 // B-spline samples at 0 and +-1 (beta(theta) = b0 + 2 b1 cos theta).
 void bspline_b01(int order, double *b0, double *b1) {
   if (order == 2) {
@@ -228,18 +239,30 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   HIPCHK(hipMemcpy(p->q0y, qy.data(), qy.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0y");
   HIPCHK(hipMalloc(&p->q0z, qz.size() * sizeof(R)), "hipMalloc(q0z)");
   HIPCHK(hipMemcpy(p->q0z, qz.data(), qz.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0z");
-  HIPCHK(hipMalloc(&p->twx, (size_t)g.lNx * sizeof(cpx<R>)), "hipMalloc(twx)");
-  HIPCHK(hipMalloc(&p->twy, (size_t)g.lNy * sizeof(cpx<R>)), "hipMalloc(twy)");
-  HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twx, g.lNx, 0), "twiddles x");
-  HIPCHK(odt::launch_twiddles<R>((cpx<R> *)p->twy, g.lNy, 0), "twiddles y");
+  // transform tables (Bluestein tables for lengths that are not powers of
+  // two, built in float64 and rounded once for float32 plans)
+  const size_t ntx = (size_t)odt::dft_table_len(g.lNx), nty = (size_t)odt::dft_table_len(g.lNy);
+  DevBuf scratch64;
+  if (sizeof(R) != sizeof(double) &&
+      (odt::uses_bluestein(g.lNx) || odt::uses_bluestein(g.lNy)))
+    HIPCHK(hipMalloc(&scratch64.p, std::max(ntx, nty) * sizeof(cpx<double>)),
+           "hipMalloc(table scratch)");
+  cpx<double> *s64 = (cpx<double> *)scratch64.p;
+  HIPCHK(hipMalloc(&p->twx, ntx * sizeof(cpx<R>)), "hipMalloc(twx)");
+  HIPCHK(hipMalloc(&p->twy, nty * sizeof(cpx<R>)), "hipMalloc(twy)");
+  HIPCHK(odt::launch_dft_table<R>((cpx<R> *)p->twx, g.lNx, s64, 0), "twiddles x");
+  if (s64) HIPCHK(hipDeviceSynchronize(), "twiddles x");
+  HIPCHK(odt::launch_dft_table<R>((cpx<R> *)p->twy, g.lNy, s64, 0), "twiddles y");
   if (sep64) {
     HIPCHK(hipMalloc(&p->P64, P64.size() * sizeof(cpx<double>)), "hipMalloc(P64)");
     HIPCHK(hipMemcpy(p->P64, P64.data(), P64.size() * sizeof(cpx<double>),
                      hipMemcpyHostToDevice), "copy P64");
-    HIPCHK(hipMalloc(&p->twx64, (size_t)g.lNx * sizeof(cpx<double>)), "hipMalloc(twx64)");
-    HIPCHK(hipMalloc(&p->twy64, (size_t)g.lNy * sizeof(cpx<double>)), "hipMalloc(twy64)");
-    HIPCHK(odt::launch_twiddles<double>((cpx<double> *)p->twx64, g.lNx, 0), "twiddles x64");
-    HIPCHK(odt::launch_twiddles<double>((cpx<double> *)p->twy64, g.lNy, 0), "twiddles y64");
+    HIPCHK(hipMalloc(&p->twx64, ntx * sizeof(cpx<double>)), "hipMalloc(twx64)");
+    HIPCHK(hipMalloc(&p->twy64, nty * sizeof(cpx<double>)), "hipMalloc(twy64)");
+    HIPCHK(odt::launch_dft_table<double>((cpx<double> *)p->twx64, g.lNx, nullptr, 0),
+           "twiddles x64");
+    HIPCHK(odt::launch_dft_table<double>((cpx<double> *)p->twy64, g.lNy, nullptr, 0),
+           "twiddles y64");
   } else if (p->fwd64) {
     p->twx64 = p->twx;
     p->twy64 = p->twy;
@@ -260,7 +283,7 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
 }
 
 int per_block(int n, size_t esz, int cap) {
-  size_t b = 65536 / ((size_t)2 * n * esz);
+  size_t b = 65536 / ((size_t)2 * odt::lds_len(n) * esz);
   if (b < 1) b = 1;
   if (b > 16) b = 16;
   return std::max(1, std::min((int)b, cap));
@@ -581,7 +604,8 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
     return fail(ODT_ERR_INVALID, "2D plans need Ny == lNy == 1");
   const size_t esz = d->precision == ODT_FP64 ? 16 : 8;
   const size_t max_lds = 160 * 1024;
-  if (2 * (size_t)d->lNx * esz > max_lds || 2 * (size_t)d->lNy * esz > max_lds)
+  const size_t llx = (size_t)odt::lds_len(d->lNx), lly = (size_t)odt::lds_len(d->lNy);
+  if (2 * llx * esz > max_lds || 2 * lly * esz > max_lds)
     return fail(ODT_ERR_UNSUPPORTED, "transform length exceeds the LDS capacity");
 
   int dev = 0;
@@ -651,7 +675,7 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
   p->NC = std::min(p->NC, 64);
   // float64 forward stage where its transforms fit the LDS in float64
   p->fwd64 = d->precision == ODT_FP64 ||
-             (2 * (size_t)d->lNx * 16 <= max_lds && 2 * (size_t)d->lNy * 16 <= max_lds);
+             (2 * llx * 16 <= max_lds && 2 * lly * 16 <= max_lds);
   const size_t esz_f = p->fwd64 ? 16 : esz;
   p->nb_fr = per_block(g.lNx, esz_f, g.lNy);
   p->nb_fc = per_block(g.lNy, esz_f, g.Lx);
@@ -992,30 +1016,32 @@ int odt_fft_c2c(int precision, int64_t batch, int32_t n, int32_t direction,
   if (batch < 0 || n < 1 || (direction != 1 && direction != -1) || !d_in || !d_out)
     return fail(ODT_ERR_INVALID, "bad fft arguments");
   const size_t esz = precision == ODT_FP64 ? 16 : 8;
-  if (2 * (size_t)n * esz > 160 * 1024)
+  if (2 * (size_t)odt::lds_len(n) * esz > 160 * 1024)
     return fail(ODT_ERR_UNSUPPORTED, "transform length exceeds the LDS capacity");
   if (batch == 0) return ODT_OK;
   hipStream_t st = (hipStream_t)stream;
-  void *tw = nullptr;
-  HIPCHK(hipMalloc(&tw, (size_t)n * esz), "hipMalloc(tw)");
+  const size_t ntw = (size_t)odt::dft_table_len(n);
+  DevBuf tw, scratch64;
+  HIPCHK(hipMalloc(&tw.p, ntw * esz), "hipMalloc(tw)");
+  if (precision == ODT_FP32 && odt::uses_bluestein(n))
+    HIPCHK(hipMalloc(&scratch64.p, ntw * 16), "hipMalloc(table scratch)");
   const int nb = per_block(n, esz, (int)std::min<int64_t>(batch, 16));
   const int logn = ilog2_exact(n);
   hipError_t e;
   if (precision == ODT_FP64) {
-    e = odt::launch_twiddles<double>((cpx<double> *)tw, n, st);
+    e = odt::launch_dft_table<double>((cpx<double> *)tw.p, n, nullptr, st);
     if (e == hipSuccess)
       e = odt::launch_fft_batch<double>((const cpx<double> *)d_in, (cpx<double> *)d_out,
                                         batch, n, logn, direction,
-                                        (const cpx<double> *)tw, nb, st);
+                                        (const cpx<double> *)tw.p, nb, st);
   } else {
-    e = odt::launch_twiddles<float>((cpx<float> *)tw, n, st);
+    e = odt::launch_dft_table<float>((cpx<float> *)tw.p, n, (cpx<double> *)scratch64.p, st);
     if (e == hipSuccess)
       e = odt::launch_fft_batch<float>((const cpx<float> *)d_in, (cpx<float> *)d_out,
                                        batch, n, logn, direction,
-                                       (const cpx<float> *)tw, nb, st);
+                                       (const cpx<float> *)tw.p, nb, st);
   }
   hipError_t e2 = hipStreamSynchronize(st);
-  (void)hipFree(tw);
   if (e != hipSuccess) return hip_fail(e, "k_fft_batch");
   if (e2 != hipSuccess) return hip_fail(e2, "k_fft_batch sync");
   return ODT_OK;
@@ -1133,9 +1159,9 @@ AppleLayout apple_layout(int n0, int n1, int n2) {
   L.part = take(((N + 255) / 256) * 8);
   L.sum = take(8);
   L.nnan = take(4);
-  L.tw0 = take((size_t)n0 * 16);
-  L.tw1 = take((size_t)n1 * 16);
-  L.tw2 = take((size_t)n2 * 16);
+  L.tw0 = take((size_t)odt::dft_table_len(n0) * 16);
+  L.tw1 = take((size_t)odt::dft_table_len(n1) * 16);
+  L.tw2 = take((size_t)odt::dft_table_len(n2) * 16);
   const double sigma = (double)std::max(n0, std::max(n1, n2)) / 100.0;
   L.r = (int)(4.0 * sigma + 0.5);
   L.w = take((size_t)(2 * L.r + 1) * 8);
@@ -1198,7 +1224,8 @@ int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2, const
   if (method != 0 && method != 1)
     return fail(ODT_ERR_INVALID, "method must be 0 (non-negativity) or 1 (symmetric histogram)");
   const int mmax = std::max(n0, std::max(n1, n2));
-  if (2 * (size_t)mmax * 16 > 160 * 1024 || n0 > 65535)
+  const int lmax = std::max(odt::lds_len(n0), std::max(odt::lds_len(n1), odt::lds_len(n2)));
+  if (2 * (size_t)lmax * 16 > 160 * 1024 || n0 > 65535)
     return fail(ODT_ERR_UNSUPPORTED, "volume edge too long for the transforms");
   const AppleLayout L = apple_layout(n0, n1, n2);
   if (ws_bytes < L.total)
@@ -1213,9 +1240,9 @@ int odt_apple_correct(int f_precision, int32_t n0, int32_t n1, int32_t n2, const
   cpx<double> *tw0 = (cpx<double> *)(w + L.tw0), *tw1 = (cpx<double> *)(w + L.tw1),
               *tw2 = (cpx<double> *)(w + L.tw2);
   const long long N = (long long)n0 * n1 * n2;
-  HIPCHK(odt::launch_twiddles<double>(tw0, n0, st), "twiddles");
-  HIPCHK(odt::launch_twiddles<double>(tw1, n1, st), "twiddles");
-  HIPCHK(odt::launch_twiddles<double>(tw2, n2, st), "twiddles");
+  HIPCHK(odt::launch_dft_table<double>(tw0, n0, nullptr, st), "twiddles");
+  HIPCHK(odt::launch_dft_table<double>(tw1, n1, nullptr, st), "twiddles");
+  HIPCHK(odt::launch_dft_table<double>(tw2, n2, nullptr, st), "twiddles");
   const double km = (2 * M_PI * nm) / res;
   HIPCHK(odt::launch_apple_core(core, n0, n1, n2, km, st), "k_apple_core");
   HIPCHK(odt::launch_apple_init(d_f, f_precision == ODT_FP64, data, N, st), "k_apple_init");

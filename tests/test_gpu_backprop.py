@@ -103,6 +103,33 @@ def test_3d_rotate_plane_chunks_vs_oracle(nx, ny):
     assert rel_l2(got32, ref) < FP32_REL_L2
 
 
+@pytest.mark.parametrize("nx,ny", [(37, 23), (45, 31), (64, 29)])
+def test_3d_unpadded_non_power_of_two_vs_oracle(nx, ny):
+    """padding=(False, False): transforms of the detector lengths
+    themselves (primes and composites that are not powers of two), as the
+    reference's FFTW plans take them (_alg3d_bpp.py:460-478)."""
+    sino, angles = create_test_sino_3d(A=11, Nx=nx, Ny=ny)
+    kw = dict(res=3.0, nm=1.335, lD=0.5, padding=(False, False))
+    ref = O.backpropagate_3d(sino, angles, **kw)
+    got = G.backpropagate_3d(sino, angles, **kw)
+    assert got.shape == ref.shape == (nx, ny, nx)
+    assert rel_l2(got, ref) < FP64_REL_L2
+    got32 = G.backpropagate_3d(sino, angles, dtype=np.float32, **kw)
+    assert rel_l2(got32, ref) < FP32_REL_L2
+
+
+@pytest.mark.parametrize("n", [101, 250])
+def test_2d_unpadded_non_power_of_two_vs_oracle(n):
+    sino, angles = create_test_sino_2d(N=n, A=17)
+    kw = dict(res=3.0, nm=1.335, lD=0.5, padding=False)
+    ref = O.backpropagate_2d(sino, angles, **kw)
+    got = G.backpropagate_2d(sino, angles, **kw)
+    assert got.shape == ref.shape == (n, n)
+    assert rel_l2(got, ref) < FP64_REL_L2
+    got32 = G.backpropagate_2d(sino, angles, dtype=np.float32, **kw)
+    assert rel_l2(got32, ref) < FP32_REL_L2
+
+
 def test_3d_iir_prefilter_equals_fused():
     sino, angles = create_test_sino_3d(A=11, Nx=20, Ny=16)
     for order in (2, 3):

@@ -27,7 +27,13 @@ def _ptr(t):
 
 @pytest.mark.parametrize("prec", [N.ODT_FP32, N.ODT_FP64])
 @pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024,
-                               2048, 4096, 10, 22, 100])
+                               2048, 4096,
+                               # Bluestein (not powers of two, <= 2048):
+                               # small primes, composites, the largest
+                               3, 5, 7, 10, 12, 22, 97, 100, 250, 251, 1000,
+                               1023, 2047,
+                               # direct DFT beyond
+                               2049])
 def test_fft_c2c_matches_numpy(prec, n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal((7, n)) + 1j * rng.standard_normal((7, n))
