@@ -889,6 +889,22 @@ __host__ __device__ constexpr int waitcnt_vm(int n) {
   return (n & 15) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8);
 }
 
+// The same for a wave-uniform count known at run time (counts above 8 wait
+// for at most 8 outstanding operations, which is stricter).
+__device__ __forceinline__ void wait_vm(int n) {
+  switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(waitcnt_vm(0)); break;
+    case 1: __builtin_amdgcn_s_waitcnt(waitcnt_vm(1)); break;
+    case 2: __builtin_amdgcn_s_waitcnt(waitcnt_vm(2)); break;
+    case 3: __builtin_amdgcn_s_waitcnt(waitcnt_vm(3)); break;
+    case 4: __builtin_amdgcn_s_waitcnt(waitcnt_vm(4)); break;
+    case 5: __builtin_amdgcn_s_waitcnt(waitcnt_vm(5)); break;
+    case 6: __builtin_amdgcn_s_waitcnt(waitcnt_vm(6)); break;
+    case 7: __builtin_amdgcn_s_waitcnt(waitcnt_vm(7)); break;
+    default: __builtin_amdgcn_s_waitcnt(waitcnt_vm(8)); break;
+  }
+}
+
 // First window row (or column) for the coordinate range [lo, hi] of a tile:
 // the spline start of the lowest in-range coordinate, one sample lower for
 // the zero-weight mirrored tap that odd orders touch at the upper edge,
@@ -978,7 +994,11 @@ struct RotTile {
   static constexpr int BPI = 256 * VEC;          // bytes per wave load
   static constexpr int K = TB / BPI / NW;        // loads per wave per angle
   static constexpr int RPI = BPI / ROWB;         // rows per wave load (VEC 4)
-  static constexpr int NLO = VEC == 4 ? (NG / RPI < K ? NG / RPI : K) : 1;
+  static constexpr int PL = RPI > 0 ? kRotWin / RPI : 1;  // loads per plane (VEC 4)
+  // 16-byte loads whose RPI rows start at multiples of 4 (samples of at
+  // most 8 bytes): window rows are placed densely from the extent on
+  static constexpr bool DENSE = VEC == 4 && RPI >= 4 && RPI % 4 == 0;
+  static constexpr int NLO = DENSE ? 2 : (VEC == 4 ? (NG / RPI < K ? NG / RPI : K) : 1);
   static_assert(ELB <= 16 && 16 % ELB == 0 && NG >= 8, "sample size");
   static_assert(TB % (BPI * NW) == 0, "tile must split evenly");
   static_assert(K <= 63, "vmcnt range");
@@ -989,38 +1009,54 @@ struct RotTile {
     return ((unsigned)r & 3u) << 1;
   }
 
-  // Per-lane byte offsets of the 16-byte loads (row within the load and
-  // permuted granule); they repeat every NLO loads.
+  // Per-lane constants of the 16-byte loads: lane l covers row l / NG of a
+  // load's RPI rows and granule position l mod NG, which holds the plane's
+  // granule (l mod NG) ^ swz(row).  DENSE: loads start at rows that are
+  // multiples of 4, so that granule depends on the lane only; lo[0] is the
+  // row's byte offset in the plane, lo[1] the granule.  Otherwise lo[i] is
+  // the lane's byte offset in load i (they repeat every NLO loads).
   __device__ static __forceinline__ void lane_offsets(unsigned *lo, int wave,
                                                       int lane, int nx) {
     if (VEC != 4) return;
     const int rl = lane / NG, g = lane % NG;
+    if constexpr (DENSE) {
+      lo[0] = (unsigned)(rl * nx) * ELB;
+      lo[1] = (unsigned)g ^ swz(rl);
+    } else {
 #pragma unroll
-    for (int i = 0; i < NLO; ++i) {
-      const int r = (wave * K + i) * RPI + rl;
-      lo[i] = (unsigned)(rl * nx + (int)(g ^ swz(r)) * AL) * ELB;
+      for (int i = 0; i < NLO; ++i) {
+        const int r = (wave * K + i) * RPI + rl;
+        lo[i] = (unsigned)(rl * nx + (int)(g ^ swz(r)) * AL) * ELB;
+      }
     }
   }
 
-  // Copy the windows of the block's planes for one angle into dst.  Load k
-  // (= wave*K + i) of the block fills tile bytes [k*BPI, (k+1)*BPI), lane l
-  // the 4*VEC bytes at k*BPI + 4*VEC*l.  16-byte loads are used only when
-  // both plane edges are >= 32, so windows lie inside the plane: the per-lane
-  // part of the address is fixed and the rest is uniform.  4-byte loads clamp
-  // rows and columns into the plane (slots outside it receive samples that no
-  // tap reads).  The two forms are separate instantiations: a branch between
-  // them lets the compiler merge their last loads, which loses the LDS
-  // object of the merged load and makes every later LDS read wait for all
-  // loads in flight.
-  // Slots outside the window extent `ext` (see k_rot_windows) load the
-  // block's first sample(s) instead (one cache line for the whole wave): every
-  // load instruction is still issued, so the vmcnt waits keep counting K
-  // per wave, and every slot holds a finite value for the zero-weight reads
-  // of out-of-range voxels.
-  __device__ static __forceinline__ void issue(const V *coefA, int wz0, int wx0,
-                                               int ext, char *dst, size_t plane,
-                                               int nyv, int nz, int nx, int wave,
-                                               int lane, const unsigned *lo) {
+  // Copy the windows of the block's planes for one angle into dst with
+  // direct global->LDS loads, 4*VEC bytes per lane, fetching only the window
+  // extent `ext` (see k_rot_windows).  16-byte loads are used only when both
+  // plane edges are >= 32, so windows lie inside the plane.  DENSE: the
+  // block's loads of a plane cover RPI consecutive rows each, from the
+  // extent's first row (aligned down to 4) on, the loads past the extent's
+  // last row repeating its last row block (same source and destination, no
+  // new cache lines); a lane whose granule lies outside the extent's
+  // columns fetches the nearest one inside instead (a slot no tap reads; the
+  // same cache lines).  The global address is a uniform base plus a fixed
+  // lane offset.  Every window row from the extent's first on then holds
+  // plane samples; rows above it are never read (the stand-in taps of
+  // voxels outside the plane use the first extent row, see prep).
+  // Otherwise load k (= wave*K + i) of the block fills tile bytes
+  // [k*BPI, (k+1)*BPI), and slots outside the extent load the block's first
+  // sample(s) instead (one cache line for the whole wave), so every slot
+  // holds a finite value.  4-byte loads also clamp rows and columns into
+  // the plane (slots outside it receive samples that no tap reads).  Every
+  // load instruction is issued, so the vmcnt waits keep counting K per wave.
+  // The two forms are separate instantiations: a branch between them lets
+  // the compiler merge their last loads, which loses the LDS object of the
+  // merged load and makes every later LDS read wait for all loads in flight.
+  __device__ static __forceinline__ int issue(const V *coefA, int wz0, int wx0,
+                                              int ext, char *dst, size_t plane,
+                                              int nyv, int nz, int nx, int wave,
+                                              int lane, const unsigned *lo) {
     const char *src = reinterpret_cast<const char *>(coefA);
     // the mask bounds the destination offsets for the compiler: it then
     // knows these stores stay inside dst and does not wait for them before
@@ -1028,7 +1064,10 @@ struct RotTile {
     wave &= NW - 1;
     const int rlo = ext & 0xff, rhi = (ext >> 8) & 0xff;
     const int clo = (ext >> 16) & 0xff, chi = (ext >> 24) & 0xff;
-    if constexpr (VEC == 4) {
+    if constexpr (DENSE) {
+      const Dense d = dense(wave, nyv, plane);
+      return issue_dense(src, wz0, wx0, ext, dst, d, nx, lo);
+    } else if constexpr (VEC == 4) {
       const unsigned uo = (unsigned)(wz0 * nx + wx0) * ELB;
       const int rl = lane / NG, gl = lane % NG;
 #pragma unroll
@@ -1069,6 +1108,61 @@ struct RotTile {
                                          4, 0, 0);
       }
     }
+    return K;
+  }
+
+  // DENSE copies: every load of a wave fills rows of one plane q; its byte
+  // offset from the angle's first plane (plane 0 when q does not exist),
+  // its LDS offset and the wave's first load index within the plane.
+  // Loop-invariant: kernels form it once per buffer.
+  struct Dense {
+    size_t qoff;
+    int dq, kkb;
+  };
+  static_assert(!DENSE || PL % K == 0, "a wave's loads stay in one plane");
+  __device__ static __forceinline__ Dense dense(int wave, int nyv, size_t plane) {
+    wave &= NW - 1;
+    const int q = wave * K / PL;
+    Dense d;
+    d.qoff = (size_t)(q < nyv ? q : 0) * plane * ELB;
+    d.dq = q * PLB;
+    d.kkb = wave * K % PL;
+    return d;
+  }
+
+  // Returns the number of loads this wave issued (wave-uniform, 0..K).
+  __device__ static __forceinline__ int issue_dense(const char *src, int wz0, int wx0,
+                                                    int ext, char *dst, const Dense &d,
+                                                    int nx, const unsigned *lo) {
+    const int rlo = ext & 0xff, rhi = (ext >> 8) & 0xff;
+    const int clo = (ext >> 16) & 0xff, chi = (ext >> 24) & 0xff;
+    const unsigned g0 = (unsigned)clo / AL, g1 = (unsigned)chi / AL;
+    unsigned gc = lo[1] < g0 ? g0 : lo[1];
+    gc = gc > g1 ? g1 : gc;
+    const unsigned voff = lo[0] + gc * 16u;
+    const int rb0 = (rlo & ~3) + RPI * d.kkb;
+    // this wave's loads that start at or before the extent's last row (and
+    // inside the window)
+    int last = rhi < kRotWin - 1 ? rhi : kRotWin - 1;
+    last = last >= rlo ? last : (rlo & ~3);  // empty extent: one row block
+    int n = (last - rb0) / RPI + 1;
+    n = rb0 > last ? 0 : (n < K ? n : K);
+    const char *wbase = src + d.qoff + (size_t)(unsigned)(wz0 * nx + wx0) * ELB;
+    char *dq = dst + d.dq;
+    const unsigned rowb = (unsigned)nx * ELB;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      if (i < n) {
+        // a block past the window's last RPI rows starts there instead
+        // (RPI = 8: overlapping rows get the same samples twice)
+        int rb = rb0 + RPI * i;
+        rb = rb < kRotWin - RPI ? rb : kRotWin - RPI;
+        __builtin_amdgcn_global_load_lds(
+            (gbl_vptr)(wbase + (size_t)((unsigned)rb * rowb) + voff),
+            (lds_vptr)(dq + rb * ROWB), 16, 0, 0);
+      }
+    }
+    return n;
   }
 
   __device__ static __forceinline__ V tap(const char *tile, unsigned off) {
@@ -1106,27 +1200,29 @@ struct RotTile {
   // value 0) or outside the volume gets zero weights and reads in-window
   // samples.
   __device__ static __forceinline__ void compute(const char *tile, int wz0,
-                                                 int wx0, const double *p,
+                                                 int wx0, int ext, const double *p,
                                                  double oz, double ox, int nz,
                                                  int nx, bool own,
                                                  wide_t<V> *acc) {
     Taps t;
-    prep(wz0, wx0, p, oz, ox, nz, nx, own, t);
+    prep(wz0, wx0, ext, p, oz, ox, nz, nx, own, t);
     gather(tile, t, acc);
   }
 
   // The tap set of the voxel (oz, ox) for one angle (coordinates, validity,
   // weights and window offsets; the same for every plane of the window).
-  __device__ static __forceinline__ void prep(int wz0, int wx0, const double *p,
-                                              double oz, double ox, int nz,
-                                              int nx, bool own, Taps &t) {
+  // A voxel outside the rotated plane reads the taps from the window's first
+  // extent row on (zero weights; rows from there on hold plane samples).
+  __device__ static __forceinline__ void prep(int wz0, int wx0, int ext,
+                                              const double *p, double oz, double ox,
+                                              int nz, int nx, bool own, Taps &t) {
     const double cz = affine2(oz, ox, p[0], p[1], p[4]);
     const double cx = affine2(oz, ox, p[2], p[3], p[5]);
     const bool valid = own && cz >= 0.0 && cz <= (double)(nz - 1) && cx >= 0.0 &&
                        cx <= (double)(nx - 1);
     const double fz = (ORDER & 1) ? floor(cz) : floor(cz + 0.5);
     const double fx = (ORDER & 1) ? floor(cx) : floor(cx + 0.5);
-    const int sz = valid ? (int)fz - ORDER / 2 : wz0 + 1;
+    const int sz = valid ? (int)fz - ORDER / 2 : wz0 + (ext & 0xff);
     const int sx = valid ? (int)fx - ORDER / 2 : wx0 + 1;
     // coordinates and spline start in float64; the weights of the float32
     // path in float32 from the exact fractional offsets, their products
@@ -1211,6 +1307,17 @@ __device__ __forceinline__ void barrier_no_fence() {
   __builtin_amdgcn_s_waitcnt(waitcnt_lgkm0());
   __builtin_amdgcn_s_barrier();
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Zero a block's LDS window buffers once before the first window copy:
+// the copies fetch only each angle's window extent, so a slot outside it
+// then holds zero or a sample of an earlier angle, both finite, for the
+// zero-weight taps of voxels outside the rotated plane.
+__device__ __forceinline__ void zero_lds(char *buf, int bytes) {
+  for (int o = (int)threadIdx.x * 16; o < bytes; o += (int)blockDim.x * 16)
+    *reinterpret_cast<int4 *>(buf + o) = make_int4(0, 0, 0, 0);
+  __builtin_amdgcn_s_waitcnt(waitcnt_lgkm0());
+  __builtin_amdgcn_s_barrier();
 }
 
 // Order in which the blocks of one y-chunk take the tiles: along a Z-order
@@ -1299,6 +1406,8 @@ __global__ void __launch_bounds__(256)
   const V *base = coef + (size_t)y0 * plane;
   const size_t aStride = (size_t)ny * plane;
   const int4 *wt = win + ti;
+  zero_lds(bufA, RT::TB);
+  zero_lds(bufB, RT::TB);
   int4 wA = make_int4(0, 0, 1, 0), wB = make_int4(0, 0, 1, 0);
   // Software pipeline over angles, two per iteration, every prefetch issued
   // from the loop body: angle a+1 goes to B while a (in A) is computed, then
@@ -1307,27 +1416,25 @@ __global__ void __launch_bounds__(256)
   for (int a = -1; a < nb; a += 2) {
     if (a + 1 < nb) {
       wB = wt[(size_t)(a + 1) * nt];
-      RT::issue(base + (a + 1) * aStride, wB.x, wB.y, wB.z, bufB, plane, nyv, nz,
-                nx, wave, lane, lo);
-      __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+      wait_vm(RT::issue(base + (a + 1) * aStride, wB.x, wB.y, wB.z, bufB, plane, nyv,
+                        nz, nx, wave, lane, lo));
     } else {
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     }
     barrier_no_fence();
     if (a >= 0)
-      RT::compute(bufA, wA.x, wA.y, params + 6 * a, oz, ox, nz, nx, own, sum);
+      RT::compute(bufA, wA.x, wA.y, wA.z, params + 6 * a, oz, ox, nz, nx, own, sum);
     barrier_no_fence();
     if (a + 1 < nb) {
       if (a + 2 < nb) {
         wA = wt[(size_t)(a + 2) * nt];
-        RT::issue(base + (a + 2) * aStride, wA.x, wA.y, wA.z, bufA, plane, nyv,
-                  nz, nx, wave, lane, lo);
-        __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+        wait_vm(RT::issue(base + (a + 2) * aStride, wA.x, wA.y, wA.z, bufA, plane, nyv,
+                          nz, nx, wave, lane, lo));
       } else {
         __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
       }
       barrier_no_fence();
-      RT::compute(bufB, wB.x, wB.y, params + 6 * (a + 1), oz, ox, nz, nx, own,
+      RT::compute(bufB, wB.x, wB.y, wB.z, params + 6 * (a + 1), oz, ox, nz, nx, own,
                   sum);
       barrier_no_fence();
     }
@@ -1415,35 +1522,49 @@ __global__ void __launch_bounds__(256)
   const V *base1 = nyv1 > 0 ? base0 + 2 * plane : base0;
   const size_t aStride = (size_t)ny * plane;
   const int4 *wt = win + ti;
+  typename RT::Dense d0{}, d1{};
+  if constexpr (RT::DENSE) {
+    d0 = RT::dense(wave, nyv0, plane);
+    d1 = RT::dense(wave, nyv1, plane);
+  }
+  // window copy of one half (lower: planes 0-1, upper: 2-3)
+  // window copy of one half (lower: planes 0-1, upper: 2-3); returns the
+  // wave's load count
+  auto copy = [&](const V *b, const int4 &w, char *dst, bool upper) {
+    if constexpr (RT::DENSE)
+      return RT::issue_dense(reinterpret_cast<const char *>(b), w.x, w.y, w.z, dst,
+                             upper ? d1 : d0, nx, lo);
+    else
+      return RT::issue(b, w.x, w.y, w.z, dst, plane, upper ? nyv1 : nyv0, nz, nx, wave,
+                       lane, lo);
+  };
+  zero_lds(h0, RT::TB);
+  zero_lds(h1, RT::TB);
   int4 w = wt[0];  // nb >= 1 (the launcher checks)
-  RT::issue(base0, w.x, w.y, w.z, h0, plane, nyv0, nz, nx, wave, lane, lo);
+  copy(base0, w, h0, false);
   // angles 0 .. nb-2 prefetch the next angle's lower half; the last angle
   // (peeled) drains instead
   for (int a = 0; a + 1 < nb; ++a) {
-    RT::issue(base1 + a * aStride, w.x, w.y, w.z, h1, plane, nyv1, nz, nx, wave,
-              lane, lo);
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+    // the loads just issued may stay in flight; the earlier ones have landed
+    wait_vm(copy(base1 + a * aStride, w, h1, true));
     barrier_no_fence();
     typename RT::Taps t;
-    RT::prep(w.x, w.y, params + 6 * a, oz, ox, nz, nx, own, t);
+    RT::prep(w.x, w.y, w.z, params + 6 * a, oz, ox, nz, nx, own, t);
     RT::gather(h0, t, sum);
     barrier_no_fence();
     w = wt[(size_t)(a + 1) * nt];
-    RT::issue(base0 + (a + 1) * aStride, w.x, w.y, w.z, h0, plane, nyv0, nz, nx,
-              wave, lane, lo);
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm(RT::K));
+    wait_vm(copy(base0 + (a + 1) * aStride, w, h0, false));
     barrier_no_fence();
     RT::gather(h1, t, sum + 2);
     barrier_no_fence();
   }
   {
     const int a = nb - 1;
-    RT::issue(base1 + a * aStride, w.x, w.y, w.z, h1, plane, nyv1, nz, nx, wave,
-              lane, lo);
+    copy(base1 + a * aStride, w, h1, true);
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     barrier_no_fence();
     typename RT::Taps t;
-    RT::prep(w.x, w.y, params + 6 * a, oz, ox, nz, nx, own, t);
+    RT::prep(w.x, w.y, w.z, params + 6 * a, oz, ox, nz, nx, own, t);
     RT::gather(h0, t, sum);
     RT::gather(h1, t, sum + 2);
   }
