@@ -2151,10 +2151,24 @@ static size_t row_lds(int tpb) {
   return ((size_t)Fft8<R, N, 1>::NBUF * tpb * lds_padded(N) + 4 * tpb) * sizeof(cpx<R>);
 }
 
+// With one detector row (2D sinograms) the rows of T[a][k][0][.] and of the
+// slab [a][0][k][.] are consecutive over the slices k, so the row stage
+// views the slices as rows (Ny' = ln, ln' = 1): blocks then take many rows
+// each instead of a single transform.
+static Geom rows_view(const Geom &g) {
+  Geom v = g;
+  if (g.Ny == 1 && g.ln > 1) {
+    v.Ny = g.ln;
+    v.ln = 1;
+  }
+  return v;
+}
+
 template <typename R, typename V, int N>
 static hipError_t row_fft8_launch(const cpx<R> *T, V *slab, const cpx<R> *twx,
-                                  const R *q0x, const Geom &g, int na,
+                                  const R *q0x, const Geom &g0, int na,
                                   int slab0, int ldt, hipStream_t st) {
+  const Geom g = rows_view(g0);
   constexpr int TT = N / 8;
   const int threads = TT < 256 ? 256 : TT;
   const int tpb = threads / TT;
@@ -2240,11 +2254,12 @@ hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
       case 4096: return row_fft8_launch<R, V, 4096>(T, slab, twx, q0x, g, na, slab0, ldt, st);
     }
   }
-  const size_t sh = lds_bytes(g.lNx, nb, sizeof(cpx<R>));
+  const Geom gv = rows_view(g);
+  const size_t sh = lds_bytes(gv.lNx, nb, sizeof(cpx<R>));
   hipError_t e = set_lds((const void *)k_row_ifft<R, V>, sh);
   if (e != hipSuccess) return e;
-  dim3 grid((g.Ny + nb - 1) / nb, g.ln, na);
-  k_row_ifft<R, V><<<grid, 256, sh, st>>>(T, slab, twx, q0x, g, nb, slab0, ldt);
+  dim3 grid((gv.Ny + nb - 1) / nb, gv.ln, na);
+  k_row_ifft<R, V><<<grid, 256, sh, st>>>(T, slab, twx, q0x, gv, nb, slab0, ldt);
   return hipGetLastError();
 }
 

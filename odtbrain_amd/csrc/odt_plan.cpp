@@ -680,7 +680,8 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
   p->nb_fr = per_block(g.lNx, esz_f, g.lNy);
   p->nb_fc = per_block(g.lNy, esz_f, g.Lx);
   p->nb_col = per_block(g.lNy, esz, g.ln);
-  p->nb_row = per_block(g.lNx, esz, g.Ny);
+  // rows of the row stage: the slices when there is one detector row
+  p->nb_row = per_block(g.lNx, esz, g.Ny == 1 ? std::max(1, g.ln) : g.Ny);
 
   size_t off = 0;
   p->off_F1 = off;
