@@ -664,11 +664,16 @@ __global__ void __launch_bounds__(256)
 // transforms it, crops with the fused x-prefilter terms and writes one slab
 // row.  Block: rows y0..y0+rb-1 of slice k, tpb rows at a time.  BAND and
 // the load/crop predication as in the column stage.
+//
+// S2 non-null (2D, rows viewed as slices): there is no T; row r of angle a is
+// S2[a][c] * gt[r][c], the support spectrum times the plan's propagator
+// table (k_prop_2d's product, formed on load).
 template <typename R, typename V, int N, bool BAND>
 __global__ void __launch_bounds__(512)
     k_row_fft8(const cpx<R> *__restrict__ T, V *__restrict__ slab,
                const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
-               Geom g, int rb, int slab0, int ldt) {
+               Geom g, int rb, int slab0, int ldt, const cpx<R> *__restrict__ S2,
+               const cpx<R> *__restrict__ gt) {
   using FFT = Fft8<R, N, +1>;
   constexpr int TT = FFT::T;
   constexpr bool WAVE = TT <= 64;
@@ -692,17 +697,26 @@ __global__ void __launch_bounds__(512)
   typename FFT::Twid tw;
   FFT::load_twiddles(tw, twx, j);
   const int iters = (rb + tpb - 1) / tpb;
+  // fused 2D rows: this angle's spectrum values of the lane's slots
+  cpx<R> s2[8];
+  if (S2) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      s2[m] = slot_used<N, BAND>(m) ? S2[(size_t)ai * g.Lx + cxc[m]] : mk<R>(0, 0);
+  }
   // the next row's support values are loaded while this row is transformed
   cpx<R> vn[8];
   auto load_row = [&](int it2, cpx<R> *dst) {
     const int r2 = it2 * tpb + t;
     const int y2 = y0 + r2;
     const bool ok2 = it2 < iters && r2 < rb && y2 < ny;
-    const cpx<R> *src = T + (((size_t)ai * ln + k) * ny + (ok2 ? y2 : 0)) * ldt;
+    const cpx<R> *src = S2 ? gt + (size_t)(ok2 ? y2 : 0) * g.Lx
+                           : T + (((size_t)ai * ln + k) * ny + (ok2 ? y2 : 0)) * ldt;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       if (!slot_used<N, BAND>(m)) continue;
-      const cpx<R> val = src[cxc[m]];
+      cpx<R> val = src[cxc[m]];
+      if (S2) val = s2[m] * val;
       const bool ok = ok2 && cxv[m];
       dst[m] = mk<R>(ok ? val.re : R(0), ok ? val.im : R(0));
     }
@@ -2167,7 +2181,8 @@ static Geom rows_view(const Geom &g) {
 template <typename R, typename V, int N>
 static hipError_t row_fft8_launch(const cpx<R> *T, V *slab, const cpx<R> *twx,
                                   const R *q0x, const Geom &g0, int na,
-                                  int slab0, int ldt, hipStream_t st) {
+                                  int slab0, int ldt, const cpx<R> *S2,
+                                  const cpx<R> *gt, hipStream_t st) {
   const Geom g = rows_view(g0);
   constexpr int TT = N / 8;
   const int threads = TT < 256 ? 256 : TT;
@@ -2187,10 +2202,10 @@ static hipError_t row_fft8_launch(const cpx<R> *T, V *slab, const cpx<R> *twx,
   if (e != hipSuccess) return e;
   if (band)
     k_row_fft8<R, V, N, true><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g,
-                                                         rb, slab0, ldt);
+                                                         rb, slab0, ldt, S2, gt);
   else
     k_row_fft8<R, V, N, false><<<grid, threads, sh, st>>>(T, slab, twx, q0x, g,
-                                                          rb, slab0, ldt);
+                                                          rb, slab0, ldt, S2, gt);
   return hipGetLastError();
 }
 
@@ -2239,19 +2254,29 @@ hipError_t launch_col_ifft(const cpx<R> *S, cpx<R> *T, const cpx<R> *twy,
   return hipGetLastError();
 }
 
+// 2D plans whose rows fuse k_prop_2d's product (S2, gt non-null in
+// launch_row_ifft): the register engine and the propagator table.
+template <typename R>
+bool row_fuses_2d(const Geom &g, bool gtab) {
+  return gtab && g.lNy == 1 && g.Ny == 1 && g.Ly == 1 && !g.fuse_y && fft8_row_ok<R>(g) &&
+         g.lNx <= 4096;
+}
+
 template <typename R, typename V>
 hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
                            const R *q0x, const Geom &g, int na, int nb,
-                           int slab0, int ldt, hipStream_t st) {
+                           int slab0, int ldt, hipStream_t st, const cpx<R> *S2,
+                           const cpx<R> *gt) {
+  if ((S2 || gt) && !(S2 && gt && row_fuses_2d<R>(g, true))) return hipErrorInvalidValue;
   if (fft8_row_ok<R>(g)) {
     switch (g.lNx) {
-      case 64: return row_fft8_launch<R, V, 64>(T, slab, twx, q0x, g, na, slab0, ldt, st);
-      case 128: return row_fft8_launch<R, V, 128>(T, slab, twx, q0x, g, na, slab0, ldt, st);
-      case 256: return row_fft8_launch<R, V, 256>(T, slab, twx, q0x, g, na, slab0, ldt, st);
-      case 512: return row_fft8_launch<R, V, 512>(T, slab, twx, q0x, g, na, slab0, ldt, st);
-      case 1024: return row_fft8_launch<R, V, 1024>(T, slab, twx, q0x, g, na, slab0, ldt, st);
-      case 2048: return row_fft8_launch<R, V, 2048>(T, slab, twx, q0x, g, na, slab0, ldt, st);
-      case 4096: return row_fft8_launch<R, V, 4096>(T, slab, twx, q0x, g, na, slab0, ldt, st);
+      case 64: return row_fft8_launch<R, V, 64>(T, slab, twx, q0x, g, na, slab0, ldt, S2, gt, st);
+      case 128: return row_fft8_launch<R, V, 128>(T, slab, twx, q0x, g, na, slab0, ldt, S2, gt, st);
+      case 256: return row_fft8_launch<R, V, 256>(T, slab, twx, q0x, g, na, slab0, ldt, S2, gt, st);
+      case 512: return row_fft8_launch<R, V, 512>(T, slab, twx, q0x, g, na, slab0, ldt, S2, gt, st);
+      case 1024: return row_fft8_launch<R, V, 1024>(T, slab, twx, q0x, g, na, slab0, ldt, S2, gt, st);
+      case 2048: return row_fft8_launch<R, V, 2048>(T, slab, twx, q0x, g, na, slab0, ldt, S2, gt, st);
+      case 4096: return row_fft8_launch<R, V, 4096>(T, slab, twx, q0x, g, na, slab0, ldt, S2, gt, st);
     }
   }
   const Geom gv = rows_view(g);
@@ -2585,10 +2610,12 @@ template hipError_t launch_fwd_cols<float, double>(
   template hipError_t launch_row_ifft<R, R>(const cpx<R> *, R *,               \
                                             const cpx<R> *, const R *,         \
                                             const Geom &, int, int, int, int,  \
-                                            hipStream_t);                      \
+                                            hipStream_t, const cpx<R> *,       \
+                                            const cpx<R> *);                   \
   template hipError_t launch_row_ifft<R, cpx<R>>(                              \
       const cpx<R> *, cpx<R> *, const cpx<R> *, const R *, const Geom &, int,  \
-      int, int, int, hipStream_t);                                             \
+      int, int, int, hipStream_t, const cpx<R> *, const cpx<R> *);             \
+  template bool row_fuses_2d<R>(const Geom &, bool);                           \
   template hipError_t launch_rotate_acc<R, R>(int, const R *, int,             \
                                               const double *, int *, R *, R *, \
                                               int, int, int, int, int,         \

@@ -61,7 +61,13 @@ hipError_t launch_prop_table(const double *pa, const PropAux<R> *aux, const R *q
 template <typename R, typename V>
 hipError_t launch_row_ifft(const cpx<R> *T, V *slab, const cpx<R> *twx,
                            const R *q0x, const Geom &g, int na, int nb,
-                           int slab0, int ldt, hipStream_t st);
+                           int slab0, int ldt, hipStream_t st,
+                           const cpx<R> *S2 = nullptr, const cpx<R> *gt = nullptr);
+// 2D plans (one detector row, propagator table present) whose row stage
+// forms the column stage's product S * g_k on load (launch_row_ifft with S2
+// and gt; no T, no column stage).
+template <typename R>
+bool row_fuses_2d(const Geom &g, bool gtab);
 // Rotate-accumulate of nb angles; win is scratch of
 // 4 * nb * rotate_window_entries(nz, nx) ints (window origin and extents per tile).
 // groups > 1 (single-plane volumes): the angles run in that many concurrent

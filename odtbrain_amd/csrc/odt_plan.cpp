@@ -139,6 +139,8 @@ struct odt_plan {
   size_t esz_c, esz_v;        // bytes of complex element / slab element
   size_t off_F1, off_S, off_T, off_slab, off_comp, off_win, off_part, ws_bytes;
   int RGG = 1;  // concurrent angle groups of the 2D rotate-accumulate
+  // 2D: the row stage forms the column stage's product on load (no T)
+  bool fuse2d = false;
   // forward stage in float64 (always for FP64 plans; for FP32 plans when
   // the transforms fit the LDS in float64); the float64 twiddles and
   // prefactor it uses (the same tables as twx/twy/P in FP64 plans)
@@ -160,6 +162,12 @@ struct odt_plan {
 };
 
 namespace {
+
+// The per-plan propagator table of the column stage fits (2 GiB at most).
+bool prop_table_fits(const Geom &g, size_t esz) {
+  const size_t tab = (size_t)g.ln * g.Lx * g.Ly * esz;
+  return tab <= ((size_t)2 << 30) && !std::getenv("ODTBRAIN_AMD_NO_PROP_TABLE");
+}
 
 template <typename R>
 int upload_tables(odt_plan *p, const odt_desc *d) {
@@ -271,9 +279,9 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   // The column stage's propagator (times the fused z-prefilter terms) is
   // the same for every angle: tabulated once per plan when it fits in
   // 2 GiB (C2: 256 x 227 x 227 complex64, 106 MB).
-  const size_t tab = (size_t)g.ln * g.Lx * g.Ly * sizeof(cpx<R>);
-  if (tab <= ((size_t)2 << 30) && !std::getenv("ODTBRAIN_AMD_NO_PROP_TABLE")) {
-    HIPCHK(hipMalloc(&p->gtab, tab), "hipMalloc(gtab)");
+  if (prop_table_fits(g, sizeof(cpx<R>))) {
+    HIPCHK(hipMalloc(&p->gtab, (size_t)g.ln * g.Lx * g.Ly * sizeof(cpx<R>)),
+           "hipMalloc(gtab)");
     HIPCHK(odt::launch_prop_table<R>(p->prop_a, (const PropAux<R> *)p->aux,
                                      (const R *)p->q0z, g, (cpx<R> *)p->gtab, 0),
            "k_prop_table");
@@ -342,14 +350,16 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
                                            (const cpx<R> *)p->P, g, n, p->nb_fc, st)),
                "k_fwd_cols");
       }
-      HIPCHK(odt::launch_col_ifft<R>(S, T, (const cpx<R> *)p->twy, p->prop_a,
-                                     (const PropAux<R> *)p->aux, (const R *)p->q0z,
-                                     (const R *)p->q0y, g, n, p->nb_col, p->ldt,
-                                     (const cpx<R> *)p->gtab, st),
-             "k_col_ifft");
-      HIPCHK((odt::launch_row_ifft<R, V>(T, slab, (const cpx<R> *)p->twx,
-                                         (const R *)p->q0x, g, n, p->nb_row, c0 - b0,
-                                         p->ldt, st)),
+      if (!p->fuse2d)
+        HIPCHK(odt::launch_col_ifft<R>(S, T, (const cpx<R> *)p->twy, p->prop_a,
+                                       (const PropAux<R> *)p->aux, (const R *)p->q0z,
+                                       (const R *)p->q0y, g, n, p->nb_col, p->ldt,
+                                       (const cpx<R> *)p->gtab, st),
+               "k_col_ifft");
+      HIPCHK((odt::launch_row_ifft<R, V>(
+                 T, slab, (const cpx<R> *)p->twx, (const R *)p->q0x, g, n, p->nb_row,
+                 c0 - b0, p->ldt, st, p->fuse2d ? S : nullptr,
+                 p->fuse2d ? (const cpx<R> *)p->gtab : nullptr)),
              "k_row_ifft");
     }
     if (p->explicit_iir) {
@@ -683,13 +693,18 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
   // rows of the row stage: the slices when there is one detector row
   p->nb_row = per_block(g.lNx, esz, g.Ny == 1 ? std::max(1, g.ln) : g.Ny);
 
+  p->fuse2d = d->kind == ODT_KIND_2D && prop_table_fits(g, esz) &&
+              (d->precision == ODT_FP64 ? odt::row_fuses_2d<double>(g, true)
+                                        : odt::row_fuses_2d<float>(g, true)) &&
+              !std::getenv("ODTBRAIN_AMD_NO_FUSE_2D");
+
   size_t off = 0;
   p->off_F1 = off;
   off = align256(off + (size_t)p->NC * g.Lx * g.lNy * esz_f);
   p->off_S = off;
   off = align256(off + (size_t)p->NC * g.Lx * g.Ly * esz);
   p->off_T = off;
-  off = align256(off + (size_t)p->NC * g.ln * p->ldt * g.Ny * esz);
+  if (!p->fuse2d) off = align256(off + (size_t)p->NC * g.ln * p->ldt * g.Ny * esz);
   p->off_slab = off;
   off = align256(off + (size_t)p->NB * slab_angle);
   // Angles per rotate-accumulate launch.  A block keeps its output tile in
