@@ -165,9 +165,10 @@ int odt_plan_last_timing(const odt_plan *plan, double *rotate_ms,
  * the (complex-linear) map from the sinogram to the volume that execute
  * applies.  There is no counterpart in the reference, which has no
  * gradient; it is the backward pass of a differentiable reconstruction.
- * Supported: ODT_KIND_3D plans with complex output, ODT_FLAG_IIR_PREFILTER,
- * spline order 2..5, edge or no padding, padded sizes 64..512.  Uses the
- * plan's workspace (same size as execute). */
+ * Supported: ODT_KIND_3D plans with complex output, spline order 2..5
+ * (with ODT_FLAG_IIR_PREFILTER, or the fused prefilter of orders 2 and 3,
+ * whose transpose the transforms carry), edge or no padding, padded sizes
+ * 64..512.  Uses the plan's workspace (same size as execute). */
 int odt_plan_adjoint(odt_plan *plan, const void *d_gout, void *d_gusin,
                      void *d_ws, size_t ws_bytes, void *stream);
 

@@ -30,6 +30,9 @@ def padded_order(n, padfac):
     return int(max(64., 2 ** np.ceil(np.log(n * padfac) / np.log(2))))
 
 
+_PARAMS_CACHE = collections.OrderedDict()
+
+
 def planar_params(angles_deg, nz, nx):
     """Matrix and offset of ``scipy.ndimage.rotate(angle, axes=(0, 1 or 2),
     reshape=False)`` for each angle in degrees: input = M @ (o_z, o_x) + off.
@@ -37,8 +40,26 @@ def planar_params(angles_deg, nz, nx):
     Built exactly as scipy's rotate builds them (scipy/ndimage/
     _interpolation.py: ``c, s = special.cosdg(angle), special.sindg(angle)``,
     ``rot = [[c, s], [-s, c]]``, ``offset = in_center - rot @ out_center``)
-    so the float64 bits agree with the reference call.
+    so the float64 bits agree with the reference call.  Memoised on the
+    angles and the plane shape (repeated calls with the same angles, as in
+    iterative reconstructions, skip the per-angle loop); the returned array
+    is read-only.
     """
+    a = np.ascontiguousarray(angles_deg, dtype=np.float64).reshape(-1)
+    key = (a.tobytes(), int(nz), int(nx))
+    out = _PARAMS_CACHE.get(key)
+    if out is None:
+        out = _planar_params(a, nz, nx)
+        out.flags.writeable = False
+        _PARAMS_CACHE[key] = out
+        while len(_PARAMS_CACHE) > 64:
+            _PARAMS_CACHE.popitem(last=False)
+    else:
+        _PARAMS_CACHE.move_to_end(key)
+    return out
+
+
+def _planar_params(angles_deg, nz, nx):
     shape = np.asarray([nz, nx])
     out = np.empty((len(angles_deg), 6), dtype=np.float64)
     for i, ang in enumerate(angles_deg):

@@ -3,7 +3,7 @@ torch.autograd.Function around it.
 
 backpropagate_3d is linear in the sinogram (edge or no padding): f = M u
 with a complex matrix M.  ``backpropagate_3d_adjoint`` applies M^H on the
-GPU (odt_plan_adjoint: transposed rotation scatter, transposed B-spline
+GPU (odt_plan_adjoint: transposed rotation gather, transposed B-spline
 prefilter, forward transforms of every z-slice times the conjugate
 propagator summed over z, conjugate prefactor, inverse transform, padding
 folded back); ``Backpropagate3D`` uses the fast forward and this adjoint as
@@ -36,14 +36,17 @@ def _setup(A, lny, lnx, angles, res, nm, lD, weight_angles, padding, padfac,
 def backpropagate_3d_adjoint(grad_f, angles, res, nm, lD=0,
                              weight_angles=True, padding=(True, True),
                              padfac=1.75, padval="edge", intp_order=2,
-                             dtype=None, shape=None):
+                             dtype=None, shape=None, prefilter=None):
     """M^H grad_f for M = backpropagate_3d(., angles, res, nm, ...).
 
     ``grad_f``: (Nx, Ny, Nx) complex volume (numpy or GPU tensor);
     ``shape``: sinogram shape (A, Ny, Nx) (default from grad_f and angles).
     Returns the (A, Ny, Nx) complex sinogram gradient (numpy for numpy in,
     GPU tensor for a GPU tensor in).  Edge or no padding only (the linear
-    ramp padding is affine, not linear)."""
+    ramp padding is affine, not linear).  ``prefilter`` as for
+    backpropagate_3d: None takes the forward's default ("fused" for spline
+    orders 2 and 3, which shares the forward's plan and transposes its
+    fused prefilter exactly; "iir" otherwise)."""
     torch = _core._torch()
     angles = np.asarray(angles)
     A = angles.size
@@ -56,9 +59,11 @@ def backpropagate_3d_adjoint(grad_f, angles, res, nm, lD=0,
     if not (isinstance(padval, str) and padval == "edge") and \
             padding[0] and padding[1]:
         raise NotImplementedError("adjoint: padval='edge' or no padding")
+    if prefilter is None:
+        prefilter = "fused" if intp_order in (2, 3) else "iir"
     fields, weights, params, cdt, _ = _setup(
         A, lny, lnx, angles, res, nm, lD, weight_angles, padding, padfac,
-        padval, intp_order, dtype, d_g.device, "iir")
+        padval, intp_order, dtype, d_g.device, prefilter)
     plan = _core.get_plan(fields, weights, params, d_g.device)
     out = torch.empty((A, lny, lnx), dtype=d_g.dtype, device=d_g.device)
     plan.adjoint(d_g, out)

@@ -10,14 +10,19 @@
 // output gradient g, is
 //   u_a = w_a Pad^T DFT_x^H DFT_y^H conj(P) sum_z e^{-i k_z z}
 //         DFT_y IDFT_x^H ... = see the kernels below, stage by stage:
-//   k_adj_rotate     R_a^T g: the rotate gather transposed into a scatter
-//                    (same coordinates, weights and mirrored taps; atomics)
-//   k_scale_ends +   Q^T = D Q D^{-1} per axis, D = diag(1/2, 1, ..., 1, 1/2)
-//   k_iir            (whole-sample mirror extension; checked numerically)
-//   k_adj_row        forward transform along x of each zero-padded slab row,
-//                    support columns kept (IDFT_x^H = DFT_x)
+//   k_adj_rotate     R_a^T g: the rotate gather transposed (same
+//                    coordinates, weights and mirrored taps; no atomics)
+//   k_adj_fir +      Q^T = D Q D^{-1} per axis, D = diag(1/2, 1, ..., 1, 1/2)
+//   k_adj_fir_rows   (whole-sample mirror extension; checked numerically);
+//                    plans with the fused prefilter (orders 2, 3) skip them:
+//                    the next two kernels carry the fused operator's transpose
+//   k_adj_row        forward transform along x of each zero-padded slab row
+//                    (plus the fused x-prefilter's transposed boundary
+//                    terms), support columns kept (IDFT_x^H = DFT_x)
 //   k_adj_col        forward transform along y of each support column and
-//                    each z, times e^{-i k_z z}, summed over z in registers
+//                    each z, times the conjugate column-stage propagator
+//                    (the plan's table: e^{i k_z z}, fused z-prefilter terms
+//                    included), summed over z in registers
 //   k_adj_inv_col    times conj(P), inverse transform along y, Pad^T in y
 //   k_adj_inv_row    inverse transform along x, Pad^T in x, times w_a
 // Transforms use the register radix-8 engine for lengths 64..512 (one
@@ -257,10 +262,17 @@ __device__ __forceinline__ int sfreq(int bin) {
 // Forward transform along x of slab rows (a, y, z) placed at padxl in a
 // zero row of lNx = N; support columns -> Tp[a][z][y][cx] (stride ldt).
 // Block: rows y0..y0+rb-1 of slice k of angle a, tpb rows at a time.
+// Fused x-prefilter plans (g.fuse_x): the forward row stage crops the
+// inverse transform h and adds q0x[x] R0 + q0x[nx-1-x] R1 with
+// R0 = b1 (h[padxl-1] - h[padxl+1]), R1 = b1 (h[padxl+nx] - h[padxl+nx-2]);
+// its transpose adds b1 <q0x, row> at padxl-1 (minus at padxl+1) and
+// b1 <reversed q0x, row> at padxl+nx (minus at padxl+nx-2) to the
+// zero-padded row before the transform (q0x and b1 are real).
 template <typename R, int N>
 __global__ void __launch_bounds__(256)
     k_adj_row(const cpx<R> *__restrict__ slab, cpx<R> *__restrict__ Tp,
-              const cpx<R> *__restrict__ twx, Geom g, int rb, int ldt) {
+              const cpx<R> *__restrict__ twx, Geom g, int rb, int ldt,
+              const R *__restrict__ q0x) {
   using FFT = Fft8<R, N, -1>;
   constexpr int TT = FFT::T;
   static_assert(TT <= 64, "one transform per wavefront at most");
@@ -281,6 +293,34 @@ __global__ void __launch_bounds__(256)
       const int x = j + m * TT - g.padxl;
       v[m] = (x >= 0 && x < nx) ? src[x] : mk<R>(0, 0);
     }
+    if (g.fuse_x) {
+      cpx<R> s0 = mk<R>(0, 0), s1 = mk<R>(0, 0);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int x = j + m * TT - g.padxl;
+        if (x >= 0 && x < nx) {
+          s0 = s0 + q0x[x] * v[m];
+          s1 = s1 + q0x[nx - 1 - x] * v[m];
+        }
+      }
+      // sums over the TT lanes of this transform
+#pragma unroll
+      for (int o = 1; o < TT; o <<= 1) {
+        s0.re += __shfl_xor(s0.re, o, TT);
+        s0.im += __shfl_xor(s0.im, o, TT);
+        s1.re += __shfl_xor(s1.re, o, TT);
+        s1.im += __shfl_xor(s1.im, o, TT);
+      }
+      const R b1 = (R)g.b1;
+      const int pos[4] = {wrap(g.padxl - 1, N), wrap(g.padxl + 1, N), wrap(g.padxl + nx, N),
+                          wrap(g.padxl + nx - 2, N)};
+      const cpx<R> add[4] = {b1 * s0, (-b1) * s0, b1 * s1, (-b1) * s1};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          if (pos[q] == j + m * TT) v[m] = v[m] + add[q];
+    }
     FFT::run(v, b0, nullptr, tw, j);
     cpx<R> *dst = Tp + (((size_t)ai * ln + k) * ny + y) * ldt;
 #pragma unroll
@@ -292,14 +332,16 @@ __global__ void __launch_bounds__(256)
 }
 
 // Per support column cx of angle a: for every slice z, forward transform
-// along y of Tp[a][z][0..ny)[cx] placed at padyl, times e^{-i k_z(c,cy) z},
-// summed over z; support rows -> Sp[a][cx][cy].  One column per transform
-// slot.
+// along y of Tp[a][z][0..ny)[cx] placed at padyl, times the conjugate
+// column-stage propagator, summed over z; support rows -> Sp[a][cx][cy].
+// The propagator is the plan's table gt[k][cx][cy] when present (with the
+// fused z-prefilter terms of fused plans), else e^{i k_z(c,cy) z}.  One
+// column per transform slot.
 template <typename R, int N>
 __global__ void __launch_bounds__(256)
     k_adj_col(const cpx<R> *__restrict__ Tp, cpx<R> *__restrict__ Sp,
               const cpx<R> *__restrict__ twy, const double *__restrict__ prop_a,
-              Geom g, int ldt) {
+              Geom g, int ldt, const cpx<R> *__restrict__ gt) {
   using FFT = Fft8<R, N, -1>;
   constexpr int TT = FFT::T;
   static_assert(TT <= 64, "one transform per wavefront at most");
@@ -334,10 +376,17 @@ __global__ void __launch_bounds__(256)
     }
     FFT::run(v, b0, nullptr, tw, j);
     const double z = g.z0 + (double)k;
+    const cpx<R> *gk = gt ? gt + ((size_t)k * g.Lx + cs) * g.Ly : nullptr;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       if (!cyv[m]) continue;
-      const cpx<R> e = phasor(-pa[cyc[m]] * z, R(0));
+      cpx<R> e;
+      if (gk) {
+        const cpx<R> q = gk[cyc[m]];
+        e = mk<R>(q.re, -q.im);
+      } else {
+        e = phasor(-pa[cyc[m]] * z, R(0));
+      }
       acc[m] = acc[m] + v[m] * e;
     }
   }
@@ -534,11 +583,13 @@ static hipError_t adj_fft_launch(int stage, const void *in, void *out,
     const int rb = 16;
     dim3 grid((g.Ny + rb - 1) / rb, g.ln, na);
     k_adj_row<R, N><<<grid, 256, sh, st>>>((const cpx<R> *)in, (cpx<R> *)out,
-                                           (const cpx<R> *)tw, g, rb, ldt);
+                                           (const cpx<R> *)tw, g, rb, ldt,
+                                           (const R *)aux);
   } else if (stage == 1) {
     dim3 grid((g.Lx + tpb - 1) / tpb, na);
     k_adj_col<R, N><<<grid, 256, sh, st>>>((const cpx<R> *)in, (cpx<R> *)out,
-                                           (const cpx<R> *)tw, prop_a, g, ldt);
+                                           (const cpx<R> *)tw, prop_a, g, ldt,
+                                           (const cpx<R> *)aux);
   } else if (stage == 2) {
     dim3 grid((g.Lx + tpb - 1) / tpb, na);
     k_adj_inv_col<R, N><<<grid, 256, sh, st>>>(
@@ -553,8 +604,9 @@ static hipError_t adj_fft_launch(int stage, const void *in, void *out,
   return hipGetLastError();
 }
 
-// stage 0: k_adj_row (n = lNx), 1: k_adj_col (n = lNy), 2: k_adj_inv_col
-// (n = lNy, aux = P), 3: k_adj_inv_row (n = lNx).
+// stage 0: k_adj_row (n = lNx, aux = q0x for fused plans), 1: k_adj_col
+// (n = lNy, aux = the propagator table or null), 2: k_adj_inv_col (n = lNy,
+// aux = P), 3: k_adj_inv_row (n = lNx).
 template <typename R>
 hipError_t launch_adj_fft(int stage, int n, const void *in, void *out,
                           const void *tw, const void *aux, const double *prop_a,

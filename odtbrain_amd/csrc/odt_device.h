@@ -172,6 +172,12 @@ __device__ __forceinline__ cpx<R> *lds_transform(cpx<R> *x, cpx<R> *y, int n,
   return lds_dft<R>(x, y, n, nb, tw, dir);
 }
 
+// Index j taken modulo n into [0, n).
+__device__ __forceinline__ int wrap(int j, int n) {
+  j %= n;
+  return j < 0 ? j + n : j;
+}
+
 // ---------------------------------------------------------------------------
 // scipy.ndimage resampling rules (restated in oracle/bspline.py).
 // ---------------------------------------------------------------------------

@@ -106,11 +106,6 @@ __device__ __forceinline__ cpx<C> padded_value(const cpx<RI> *u, const Geom &g,
   return ramp_row<C>(u, g, w, end, j, i - g.padxl);
 }
 
-__device__ __forceinline__ int wrap(int j, int n) {
-  j %= n;
-  return j < 0 ? j + n : j;
-}
-
 // ---------------------------------------------------------------------------
 // forward transforms.  RI is the real type of the caller's projections and
 // of the support spectrum S; C the type the pad, both transforms and the

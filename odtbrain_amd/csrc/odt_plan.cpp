@@ -482,10 +482,13 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
   cpx<R> *gu = (cpx<R> *)d_gusin;
   const long long Nx = g.Nx, Ny = g.Ny, ln = g.ln;
   const size_t per_angle = (size_t)Ny * ln * Nx;
-  // With room for two half batches the transposed prefilter runs as two
-  // out-of-place convolutions (z into the second half, x back); otherwise
-  // as the in-place recursions.
-  const bool fir = p->NB >= 2;
+  // Fused plans (orders 2, 3) transpose the prefilter inside the
+  // transforms (k_adj_row's boundary terms, the table's conjugate in
+  // k_adj_col).  Otherwise, with room for two half batches the transposed
+  // prefilter runs as two out-of-place convolutions (z into the second
+  // half, x back), else as the in-place recursions.
+  const bool fused = !p->explicit_iir;
+  const bool fir = !fused && p->NB >= 2;
   const int step = fir ? p->NB / 2 : p->NB;
   cpx<R> *slabB = slab + (size_t)step * per_angle;
   if (fir && !p->fir_taps)
@@ -495,7 +498,9 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
     HIPCHK(odt::launch_adj_rotate<R>(p->order, gout, nb, p->params + 6 * (size_t)b0,
                                      slab, g.Ny, g.ln, g.Nx, st),
            "k_adj_rotate");
-    if (fir) {
+    if (fused) {
+      // nothing: the transforms below carry the transposed prefilter
+    } else if (fir) {
       const R *hz = (const R *)p->fir_taps;
       HIPCHK(odt::launch_adj_fir<R>(slab, slabB, nb * Ny * Nx, g.ln, Nx, 1, ln * Nx, Nx,
                                     hz, p->fir_K, st),
@@ -518,9 +523,10 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
     for (int c0 = b0; c0 < b0 + nb; c0 += p->NC) {
       const int n = std::min(p->NC, b0 + nb - c0);
       HIPCHK(odt::launch_adj_fft<R>(0, g.lNx, slab + (size_t)(c0 - b0) * per_angle, Tp,
-                                    p->twx, nullptr, nullptr, nullptr, 0, g, n, p->ldt, st),
+                                    p->twx, g.fuse_x ? p->q0x : nullptr, nullptr, nullptr,
+                                    0, g, n, p->ldt, st),
              "k_adj_row");
-      HIPCHK(odt::launch_adj_fft<R>(1, g.lNy, Tp, Sp, p->twy, nullptr, p->prop_a, nullptr,
+      HIPCHK(odt::launch_adj_fft<R>(1, g.lNy, Tp, Sp, p->twy, p->gtab, p->prop_a, nullptr,
                                     0, g, n, p->ldt, st),
              "k_adj_col");
       HIPCHK(odt::launch_adj_fft<R>(2, g.lNy, Sp, U1, p->twy, p->P, nullptr, nullptr, 0, g,
@@ -845,11 +851,14 @@ int odt_plan_destroy(odt_plan *p) {
 int odt_plan_adjoint(odt_plan *p, const void *d_gout, void *d_gusin, void *ws,
                      size_t ws_bytes, void *stream) {
   if (!p || !d_gout || !d_gusin) return fail(ODT_ERR_INVALID, "null argument");
-  if (p->kind != ODT_KIND_3D || p->onlyreal || !p->explicit_iir || p->order < 2 ||
+  if (p->kind != ODT_KIND_3D || p->onlyreal || p->order < 2 ||
       (p->g.pad_mode != ODT_PAD_NONE && p->g.pad_mode != ODT_PAD_EDGE))
     return fail(ODT_ERR_UNSUPPORTED,
-                "adjoint: y-axis 3D plans with complex output, the explicit "
-                "prefilter, spline order 2..5 and edge or no padding only");
+                "adjoint: y-axis 3D plans with complex output, spline order "
+                "2..5 and edge or no padding only");
+  if (!p->explicit_iir && (!p->gtab || p->g.fuse_y))
+    return fail(ODT_ERR_UNSUPPORTED,
+                "adjoint: fused-prefilter plans need their propagator table");
   auto pow2_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
   if (!pow2_ok(p->g.lNx) || !pow2_ok(p->g.lNy))
     return fail(ODT_ERR_UNSUPPORTED, "adjoint: padded sizes 64..512 (powers of two) only");

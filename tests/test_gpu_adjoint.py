@@ -49,6 +49,24 @@ def test_adjoint_dot_product_fp64(order, case):
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
 
 
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("pad", [True, False])
+def test_fused_adjoint_equals_explicit_prefilter_adjoint(order, pad):
+    """The transposed fused prefilter (boundary terms in the x transform,
+    conjugate fused propagator in the z sum) against the transposed
+    recursive prefilter (FIR convolutions): the same operator."""
+    A, ny, nx = (9, 20, 24) if pad else (7, 64, 64)
+    kw = {} if pad else dict(padding=(False, False))
+    angles = np.linspace(0.3, 2 * np.pi + 0.3, A, endpoint=False)
+    g = _rand((nx, ny, nx), np.complex128, 30 + order)
+    a = AG.backpropagate_3d_adjoint(g, angles, 6.0, 1.333, intp_order=order,
+                                    dtype=np.float64, prefilter="fused", **kw)
+    b = AG.backpropagate_3d_adjoint(g, angles, 6.0, 1.333, intp_order=order,
+                                    dtype=np.float64, prefilter="iir", **kw)
+    err = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    assert err < 1e-10, err
+
+
 def test_adjoint_dot_product_fp32():
     A, ny, nx = 16, 30, 32
     angles = np.linspace(0, 2 * np.pi, A, endpoint=False)
