@@ -283,6 +283,25 @@ __global__ void __launch_bounds__(256)
   const int ln = g.ln, ny = g.Ny, nx = g.Nx;
   typename FFT::Twid tw;
   FFT::load_twiddles(tw, twx, j);
+  // fused x-prefilter: the lane's q0x weights and, per slot, the signed
+  // factor (+-b1 or 0) of the two boundary sums it receives (the same for
+  // every row)
+  R qa[8], qb[8], fa[8], fb[8];
+  const bool fx = g.fuse_x;
+  if (fx) {
+    const R b1 = (R)g.b1;
+    const int p0 = wrap(g.padxl - 1, N), p1 = wrap(g.padxl + 1, N);
+    const int p2 = wrap(g.padxl + nx, N), p3 = wrap(g.padxl + nx - 2, N);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int x = j + m * TT - g.padxl, pm = j + m * TT;
+      const bool in = x >= 0 && x < nx;
+      qa[m] = in ? q0x[x] : R(0);
+      qb[m] = in ? q0x[nx - 1 - x] : R(0);
+      fa[m] = (pm == p0 ? b1 : R(0)) - (pm == p1 ? b1 : R(0));
+      fb[m] = (pm == p2 ? b1 : R(0)) - (pm == p3 ? b1 : R(0));
+    }
+  }
   for (int r = t; r < rb; r += tpb) {
     const int y = y0 + r;
     if (y >= ny) break;
@@ -293,15 +312,12 @@ __global__ void __launch_bounds__(256)
       const int x = j + m * TT - g.padxl;
       v[m] = (x >= 0 && x < nx) ? src[x] : mk<R>(0, 0);
     }
-    if (g.fuse_x) {
+    if (fx) {
       cpx<R> s0 = mk<R>(0, 0), s1 = mk<R>(0, 0);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const int x = j + m * TT - g.padxl;
-        if (x >= 0 && x < nx) {
-          s0 = s0 + q0x[x] * v[m];
-          s1 = s1 + q0x[nx - 1 - x] * v[m];
-        }
+        s0 = s0 + qa[m] * v[m];
+        s1 = s1 + qb[m] * v[m];
       }
       // sums over the TT lanes of this transform
 #pragma unroll
@@ -311,15 +327,8 @@ __global__ void __launch_bounds__(256)
         s1.re += __shfl_xor(s1.re, o, TT);
         s1.im += __shfl_xor(s1.im, o, TT);
       }
-      const R b1 = (R)g.b1;
-      const int pos[4] = {wrap(g.padxl - 1, N), wrap(g.padxl + 1, N), wrap(g.padxl + nx, N),
-                          wrap(g.padxl + nx - 2, N)};
-      const cpx<R> add[4] = {b1 * s0, (-b1) * s0, b1 * s1, (-b1) * s1};
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-          if (pos[q] == j + m * TT) v[m] = v[m] + add[q];
+      for (int m = 0; m < 8; ++m) v[m] = v[m] + fa[m] * s0 + fb[m] * s1;
     }
     FFT::run(v, b0, nullptr, tw, j);
     cpx<R> *dst = Tp + (((size_t)ai * ln + k) * ny + y) * ldt;
