@@ -299,6 +299,16 @@ int odt_apple_envelope(int32_t n0, int32_t n1, int32_t n2, const void *d_ftdata,
 const char *odt_last_error(void);
 int odt_version(void);
 
+/* Device allocations the library holds (plans' tables and the temporary
+ * buffers of running calls).  No counterpart in the reference; it is for
+ * tests of the error paths: with the environment variable
+ * ODTBRAIN_AMD_FAULT="alloc:N" the N-th device allocation after the variable
+ * took that value fails with hipErrorOutOfMemory, with "launch:N" the N-th
+ * checked launch or HIP call reports hipErrorLaunchFailure after it was
+ * issued; each fault fires once per value.  A failed call must leave this
+ * count where it was. */
+int odt_debug_live_allocations(int64_t *count);
+
 #ifdef __cplusplus
 }
 #endif

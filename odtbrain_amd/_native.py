@@ -28,7 +28,7 @@ EXPORTED = (
     "odt_prepare_workspace_size", "odt_sinogram_prepare",
     "odt_apple_workspace_size", "odt_apple_core", "odt_apple_correct",
     "odt_apple_envelope",
-    "odt_last_error", "odt_version",
+    "odt_last_error", "odt_version", "odt_debug_live_allocations",
 )
 
 
@@ -139,12 +139,23 @@ def lib():
     L.odt_last_error.restype = ctypes.c_char_p
     L.odt_last_error.argtypes = []
     L.odt_version.argtypes = []
+    L.odt_debug_live_allocations.argtypes = [ctypes.POINTER(ctypes.c_int64)]
     for name in EXPORTED:
         f = getattr(L, name)
         if name != "odt_last_error":
             f.restype = ctypes.c_int
     _lib = L
     return L
+
+
+def live_allocations():
+    """Device allocations the library holds (see odt_debug_live_allocations
+    in include/odtbrain_hip.h: the count a failed call must leave as it
+    was)."""
+    n = ctypes.c_int64(0)
+    check(lib().odt_debug_live_allocations(ctypes.byref(n)),
+          "odt_debug_live_allocations")
+    return int(n.value)
 
 
 def check(rc, what):

@@ -35,15 +35,70 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
+// Fault injection for tests of the error paths, on the host: with
+// ODTBRAIN_AMD_FAULT="alloc:N" the N-th device allocation the library makes
+// after the variable took that value reports hipErrorOutOfMemory instead of
+// allocating; with "launch:N" the N-th checked launch or HIP call reports
+// hipErrorLaunchFailure after it was issued (so the kernels themselves run
+// and end normally).  Each fault fires once per value of the variable.
+std::mutex g_fault_mu;
+std::string g_fault_spec;
+long long g_fault_count[2] = {0, 0};
+long long g_live_allocs = 0;  // device allocations of the library not yet freed
+
+bool fault_fires(int kind) {  // 0: allocation, 1: launch
+  const char *env = std::getenv("ODTBRAIN_AMD_FAULT");
+  if (!env || !*env) return false;
+  std::lock_guard<std::mutex> lk(g_fault_mu);
+  if (g_fault_spec != env) {
+    g_fault_spec = env;
+    g_fault_count[0] = g_fault_count[1] = 0;
+  }
+  const char *what = kind == 0 ? "alloc:" : "launch:";
+  const size_t wl = std::strlen(what);
+  if (g_fault_spec.compare(0, wl, what) != 0) return false;
+  const long long n = std::atoll(g_fault_spec.c_str() + wl);
+  return ++g_fault_count[kind] == n;
+}
+
+hipError_t dev_malloc(void **ptr, size_t bytes) {
+  if (fault_fires(0)) {
+    *ptr = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  const hipError_t e = hipMalloc(ptr, bytes);
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_fault_mu);
+    ++g_live_allocs;
+  }
+  return e;
+}
+template <typename T>
+hipError_t dev_malloc(T **ptr, size_t bytes) {
+  return dev_malloc(reinterpret_cast<void **>(ptr), bytes);
+}
+
+void dev_free(void *ptr) {
+  if (!ptr) return;
+  if (hipFree(ptr) == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_fault_mu);
+    --g_live_allocs;
+  }
+}
+
+hipError_t checked(hipError_t e) {
+  return e == hipSuccess && fault_fires(1) ? hipErrorLaunchFailure : e;
+}
+
 int hip_fail(hipError_t e, const char *where) {
   g_err = std::string(where) + ": " + hipGetErrorString(e);
   return ODT_ERR_HIP;
 }
 
-#define HIPCHK(expr, where)                   \
-  do {                                        \
-    hipError_t e_ = (expr);                   \
-    if (e_ != hipSuccess) return hip_fail(e_, where); \
+#define HIPCHK(expr, where)                             \
+  do {                                                  \
+    hipError_t e_ = checked(expr);                      \
+    if (e_ != hipSuccess) return hip_fail(e_, where);   \
   } while (0)
 
 int ilog2_exact(int n) {
@@ -60,7 +115,7 @@ struct DevBuf {
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p);
   }
 };
 
@@ -235,17 +290,17 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   };
   const auto qx = q0vec(g.Nx, g.fuse_x), qy = q0vec(g.Ny, g.fuse_y),
              qz = q0vec(g.ln, g.fuse_z);
-  HIPCHK(hipMalloc(&p->P, P.size() * sizeof(cpx<R>)), "hipMalloc(P)");
+  HIPCHK(dev_malloc(&p->P, P.size() * sizeof(cpx<R>)), "hipMalloc(P)");
   HIPCHK(hipMemcpy(p->P, P.data(), P.size() * sizeof(cpx<R>), hipMemcpyHostToDevice), "copy P");
-  HIPCHK(hipMalloc((void **)&p->prop_a, pa.size() * sizeof(double)), "hipMalloc(prop)");
+  HIPCHK(dev_malloc((void **)&p->prop_a, pa.size() * sizeof(double)), "hipMalloc(prop)");
   HIPCHK(hipMemcpy(p->prop_a, pa.data(), pa.size() * sizeof(double), hipMemcpyHostToDevice), "copy prop");
-  HIPCHK(hipMalloc(&p->aux, aux.size() * sizeof(PropAux<R>)), "hipMalloc(aux)");
+  HIPCHK(dev_malloc(&p->aux, aux.size() * sizeof(PropAux<R>)), "hipMalloc(aux)");
   HIPCHK(hipMemcpy(p->aux, aux.data(), aux.size() * sizeof(PropAux<R>), hipMemcpyHostToDevice), "copy aux");
-  HIPCHK(hipMalloc(&p->q0x, qx.size() * sizeof(R)), "hipMalloc(q0x)");
+  HIPCHK(dev_malloc(&p->q0x, qx.size() * sizeof(R)), "hipMalloc(q0x)");
   HIPCHK(hipMemcpy(p->q0x, qx.data(), qx.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0x");
-  HIPCHK(hipMalloc(&p->q0y, qy.size() * sizeof(R)), "hipMalloc(q0y)");
+  HIPCHK(dev_malloc(&p->q0y, qy.size() * sizeof(R)), "hipMalloc(q0y)");
   HIPCHK(hipMemcpy(p->q0y, qy.data(), qy.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0y");
-  HIPCHK(hipMalloc(&p->q0z, qz.size() * sizeof(R)), "hipMalloc(q0z)");
+  HIPCHK(dev_malloc(&p->q0z, qz.size() * sizeof(R)), "hipMalloc(q0z)");
   HIPCHK(hipMemcpy(p->q0z, qz.data(), qz.size() * sizeof(R), hipMemcpyHostToDevice), "copy q0z");
   // transform tables (Bluestein tables for lengths that are not powers of
   // two, built in float64 and rounded once for float32 plans)
@@ -253,20 +308,20 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   DevBuf scratch64;
   if (sizeof(R) != sizeof(double) &&
       (odt::uses_bluestein(g.lNx) || odt::uses_bluestein(g.lNy)))
-    HIPCHK(hipMalloc(&scratch64.p, std::max(ntx, nty) * sizeof(cpx<double>)),
+    HIPCHK(dev_malloc(&scratch64.p, std::max(ntx, nty) * sizeof(cpx<double>)),
            "hipMalloc(table scratch)");
   cpx<double> *s64 = (cpx<double> *)scratch64.p;
-  HIPCHK(hipMalloc(&p->twx, ntx * sizeof(cpx<R>)), "hipMalloc(twx)");
-  HIPCHK(hipMalloc(&p->twy, nty * sizeof(cpx<R>)), "hipMalloc(twy)");
+  HIPCHK(dev_malloc(&p->twx, ntx * sizeof(cpx<R>)), "hipMalloc(twx)");
+  HIPCHK(dev_malloc(&p->twy, nty * sizeof(cpx<R>)), "hipMalloc(twy)");
   HIPCHK(odt::launch_dft_table<R>((cpx<R> *)p->twx, g.lNx, s64, 0), "twiddles x");
   if (s64) HIPCHK(hipDeviceSynchronize(), "twiddles x");
   HIPCHK(odt::launch_dft_table<R>((cpx<R> *)p->twy, g.lNy, s64, 0), "twiddles y");
   if (sep64) {
-    HIPCHK(hipMalloc(&p->P64, P64.size() * sizeof(cpx<double>)), "hipMalloc(P64)");
+    HIPCHK(dev_malloc(&p->P64, P64.size() * sizeof(cpx<double>)), "hipMalloc(P64)");
     HIPCHK(hipMemcpy(p->P64, P64.data(), P64.size() * sizeof(cpx<double>),
                      hipMemcpyHostToDevice), "copy P64");
-    HIPCHK(hipMalloc(&p->twx64, ntx * sizeof(cpx<double>)), "hipMalloc(twx64)");
-    HIPCHK(hipMalloc(&p->twy64, nty * sizeof(cpx<double>)), "hipMalloc(twy64)");
+    HIPCHK(dev_malloc(&p->twx64, ntx * sizeof(cpx<double>)), "hipMalloc(twx64)");
+    HIPCHK(dev_malloc(&p->twy64, nty * sizeof(cpx<double>)), "hipMalloc(twy64)");
     HIPCHK(odt::launch_dft_table<double>((cpx<double> *)p->twx64, g.lNx, nullptr, 0),
            "twiddles x64");
     HIPCHK(odt::launch_dft_table<double>((cpx<double> *)p->twy64, g.lNy, nullptr, 0),
@@ -280,7 +335,7 @@ int upload_tables(odt_plan *p, const odt_desc *d) {
   // the same for every angle: tabulated once per plan when it fits in
   // 2 GiB (C2: 256 x 227 x 227 complex64, 106 MB).
   if (prop_table_fits(g, sizeof(cpx<R>))) {
-    HIPCHK(hipMalloc(&p->gtab, (size_t)g.ln * g.Lx * g.Ly * sizeof(cpx<R>)),
+    HIPCHK(dev_malloc(&p->gtab, (size_t)g.ln * g.Lx * g.Ly * sizeof(cpx<R>)),
            "hipMalloc(gtab)");
     HIPCHK(odt::launch_prop_table<R>(p->prop_a, (const PropAux<R> *)p->aux,
                                      (const R *)p->q0z, g, (cpx<R> *)p->gtab, 0),
@@ -301,10 +356,10 @@ void free_plan(odt_plan *p) {
   void *ptrs[] = {p->twx, p->twy, p->P, p->aux, p->q0x, p->q0y, p->q0z,
                   p->prop_a, p->weights, p->params, p->fir_taps, p->gtab};
   for (void *q : ptrs)
-    if (q) (void)hipFree(q);
-  if (p->P64 && p->P64 != p->P) (void)hipFree(p->P64);
-  if (p->twx64 && p->twx64 != p->twx) (void)hipFree(p->twx64);
-  if (p->twy64 && p->twy64 != p->twy) (void)hipFree(p->twy64);
+    if (q) dev_free(q);
+  if (p->P64 && p->P64 != p->P) dev_free(p->P64);
+  if (p->twx64 && p->twx64 != p->twx) dev_free(p->twx64);
+  if (p->twy64 && p->twy64 != p->twy) dev_free(p->twy64);
   for (auto e : p->ev_rot) (void)hipEventDestroy(e);
   if (p->ev_t0) (void)hipEventDestroy(p->ev_t0);
   if (p->ev_t1) (void)hipEventDestroy(p->ev_t1);
@@ -462,7 +517,7 @@ int upload_fir(odt_plan *p) {
   const int pad = odt::adj_fir_pad();
   std::vector<R> hz(2 * K + 2 * pad, R(0));
   for (int k = -K; k <= K; ++k) hz[k + K + pad] = (R)h[k + K];
-  HIPCHK(hipMalloc(&p->fir_taps, hz.size() * sizeof(R)), "hipMalloc(taps)");
+  HIPCHK(dev_malloc(&p->fir_taps, hz.size() * sizeof(R)), "hipMalloc(taps)");
   HIPCHK(hipMemcpy(p->fir_taps, hz.data(), hz.size() * sizeof(R), hipMemcpyHostToDevice),
          "hipMemcpy(taps)");
   p->fir_K = K;
@@ -573,6 +628,13 @@ int scratch_address(int dev, void **out) {
 extern "C" {
 
 const char *odt_last_error(void) { return g_err.c_str(); }
+
+int odt_debug_live_allocations(int64_t *count) {
+  if (!count) return fail(ODT_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(g_fault_mu);
+  *count = g_live_allocs;
+  return ODT_OK;
+}
 
 int odt_version(void) { return 1; }
 
@@ -764,10 +826,10 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
 
   int rc;
   hipError_t e;
-  if ((e = hipMalloc((void **)&p->weights, sizeof(double) * d->A)) != hipSuccess ||
+  if ((e = dev_malloc((void **)&p->weights, sizeof(double) * d->A)) != hipSuccess ||
       (e = hipMemcpy(p->weights, d->weights, sizeof(double) * d->A,
                      hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMalloc((void **)&p->params, sizeof(double) * d->A * want_stride)) !=
+      (e = dev_malloc((void **)&p->params, sizeof(double) * d->A * want_stride)) !=
           hipSuccess ||
       (e = hipMemcpy(p->params, d->angle_params, sizeof(double) * d->A * want_stride,
                      hipMemcpyHostToDevice)) != hipSuccess) {
@@ -1047,9 +1109,9 @@ int odt_fft_c2c(int precision, int64_t batch, int32_t n, int32_t direction,
   hipStream_t st = (hipStream_t)stream;
   const size_t ntw = (size_t)odt::dft_table_len(n);
   DevBuf tw, scratch64;
-  HIPCHK(hipMalloc(&tw.p, ntw * esz), "hipMalloc(tw)");
+  HIPCHK(dev_malloc(&tw.p, ntw * esz), "hipMalloc(tw)");
   if (precision == ODT_FP32 && odt::uses_bluestein(n))
-    HIPCHK(hipMalloc(&scratch64.p, ntw * 16), "hipMalloc(table scratch)");
+    HIPCHK(dev_malloc(&scratch64.p, ntw * 16), "hipMalloc(table scratch)");
   const int nb = per_block(n, esz, (int)std::min<int64_t>(batch, 16));
   const int logn = ilog2_exact(n);
   hipError_t e;
@@ -1398,7 +1460,7 @@ int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
   const size_t pbytes = align256(sizeof(double) * 6 * nang);
   const size_t wbytes = 4 * sizeof(int) * (size_t)nang * odt::rotate_window_entries(nz, nx);
   char *dp = nullptr;
-  HIPCHK(hipMalloc((void **)&dp, pbytes + wbytes), "hipMalloc(params)");
+  HIPCHK(dev_malloc((void **)&dp, pbytes + wbytes), "hipMalloc(params)");
   hipError_t e = hipMemcpyAsync(dp, params, sizeof(double) * 6 * nang,
                                 hipMemcpyHostToDevice, st);
   if (e == hipSuccess) {
@@ -1414,7 +1476,7 @@ int odt_rotate_accumulate(int precision, int order, int32_t nang, int32_t ny,
                                                    nz, nx, first, 1, st);
   }
   hipError_t e2 = hipStreamSynchronize(st);
-  (void)hipFree(dp);
+  dev_free(dp);
   if (e != hipSuccess) return hip_fail(e, "k_rotate_acc");
   if (e2 != hipSuccess) return hip_fail(e2, "k_rotate_acc sync");
   return ODT_OK;
