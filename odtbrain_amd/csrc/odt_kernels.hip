@@ -1673,8 +1673,8 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // LDS-staged affine gather-accumulate for the tilted axis (same arithmetic
 // as k_affine_acc).  A block (4 waves) owns an 8x8x8 output tile; wave w
-// holds the (x, z) patch of the planes y0 + 2w and y0 + 2w + 1, lane l the
-// voxel (x0 + (l & 7), z0 + (l >> 3)) of both.  Per angle the block copies
+// holds the (x, z) patch of the planes y0 + 2w and y0 + 2w + 1, in 4x4
+// quads of 16 lanes.  Per angle the block copies
 // the window of the slab that holds every tap (mirrored taps included) of
 // the tile's in-range voxels into LDS -- at most WIN = 15 + ORDER samples
 // per axis: a rotation moves a 7-sample tile edge over at most 7*sqrt(3)
@@ -1688,6 +1688,22 @@ constexpr int kAffTile = 8;
 template <int ORDER>
 __host__ __device__ constexpr int aff_win() { return 15 + ORDER; }
 
+// LDS pitches of the window box (elements): rows of 20 and planes a
+// multiple of 16 spread a wave's taps -- a rotated 4x4 voxel patch per 16
+// lanes -- over the banks (a bank model over the tilted rotations puts the
+// 8-byte reads at 1.7 passes per 16 lanes, against 3.9 for the dense W x W
+// planes).  16-byte elements keep dense planes: the padding would halve the
+// blocks per CU.
+template <int ORDER, typename V>
+__host__ __device__ constexpr int aff_row_pitch() {
+  return sizeof(V) >= 16 || aff_win<ORDER>() > 20 ? aff_win<ORDER>() : 20;
+}
+template <int ORDER, typename V>
+__host__ __device__ constexpr int aff_plane_pitch() {
+  return sizeof(V) >= 16 ? aff_win<ORDER>() * aff_win<ORDER>()
+                         : (aff_win<ORDER>() * aff_row_pitch<ORDER, V>() + 15) / 16 * 16;
+}
+
 size_t affine_window_entries(int ny, int nz, int nx) {
   return (size_t)((nx + kAffTile - 1) / kAffTile) * ((ny + kAffTile - 1) / kAffTile) *
          ((nz + kAffTile - 1) / kAffTile);
@@ -1695,8 +1711,9 @@ size_t affine_window_entries(int ny, int nz, int nx) {
 
 // Window rows [lo, hi] along one axis (edge n) for coordinates spanning
 // [clo, chi]: the taps start(c) .. start(c) + ORDER of the in-range part,
-// after one whole-sample reflection at either end; packed lo | ext << 16,
-// ext 0 when no coordinate is in range.
+// after one whole-sample reflection at either end; packed lo | ext << 16 |
+// direct << 24, ext 0 when no coordinate is in range, direct when no tap
+// needs the reflection.
 template <int ORDER>
 __device__ __forceinline__ int aff_axis(double clo, double chi, int n) {
   clo = fmax(clo, 0.0);
@@ -1708,8 +1725,10 @@ __device__ __forceinline__ int aff_axis(double clo, double chi, int n) {
   lo = lo < 0 ? 0 : lo;
   hi = hi > n - 1 ? n - 1 : hi;
   int ext = hi - lo + 1;
+  // every tap of the range inside [0, n) and inside the window: no mirror
+  const int direct = s0 >= 0 && s1 <= n - 1 && ext <= aff_win<ORDER>() ? 1 : 0;
   ext = ext < aff_win<ORDER>() ? ext : aff_win<ORDER>();
-  return lo | (ext << 16);
+  return lo | (ext << 16) | (direct << 24);
 }
 
 // Window of every (angle, tile) in the (x, -y, z) frame: the eight tile
@@ -1744,8 +1763,10 @@ __global__ void k_affine_windows(const double *__restrict__ params, int nb, int 
   const int w0 = aff_axis<ORDER>(lo[0], hi[0], nx);
   const int w1 = aff_axis<ORDER>(lo[1], hi[1], ny);
   const int w2 = aff_axis<ORDER>(lo[2], hi[2], nz);
+  const int inner = (w0 >> 24) & (w1 >> 24) & (w2 >> 24) & 1;
   win[i] = make_int4(w0 & 0xffff, w1 & 0xffff, w2 & 0xffff,
-                     (w0 >> 16) | ((w1 >> 16) << 8) | ((w2 >> 16) << 16));
+                     ((w0 >> 16) & 0xff) | (((w1 >> 16) & 0xff) << 8) |
+                         (((w2 >> 16) & 0xff) << 16) | (inner << 24));
 }
 
 template <typename R, typename V, int ORDER>
@@ -1755,8 +1776,9 @@ __global__ void __launch_bounds__(256)
                  V *__restrict__ comp, int ny, int nz, int nx, int first, int last,
                  int ntx, int nty, int ntz, int xcd_map) {
   constexpr int W = aff_win<ORDER>();
-  constexpr int NS = (W * W * W + 255) / 256;  // window slots per thread
-  __shared__ V buf[W * W * W];
+  constexpr int RP = aff_row_pitch<ORDER, V>(), PP = aff_plane_pitch<ORDER, V>();
+  constexpr int NS = (W * W * W + 255) / 256;  // window samples per thread
+  __shared__ V buf[W * PP];
   const int nt = ntx * nty * ntz;
   int ti = blockIdx.x;
   if (xcd_map) {
@@ -1765,8 +1787,10 @@ __global__ void __launch_bounds__(256)
     ti = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   }
   const int tx = ti % ntx, ty = (ti / ntx) % nty, tz = ti / (ntx * nty);
+  // lanes 16g .. 16g + 15 take the 4x4 (x, z) quad g of the 8x8 patch
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x = tx * kAffTile + (lane & 7), z = tz * kAffTile + (lane >> 3);
+  const int x = tx * kAffTile + 4 * ((lane >> 4) & 1) + (lane & 3),
+            z = tz * kAffTile + 4 * (lane >> 5) + ((lane >> 2) & 3);
   const int ya = ty * kAffTile + 2 * wave;
   bool own[2];
   size_t o[2];
@@ -1780,39 +1804,42 @@ __global__ void __launch_bounds__(256)
     if (!first && own[k]) sum[k] = load_sum(out, comp, o[k]);
   }
   const size_t vol = (size_t)ny * nz * nx;
-  // The window as a W x W x W box in LDS, slot (j, k, i) = sample
-  // (lo0 + i, lo1 + j, lo2 + k); a thread takes the slots t, t + 256, ...
-  // Per slot the part of the sample offset that does not depend on the
-  // window origin, -j*nz*nx + k*nx + i, and the packed (i, j, k) are formed
-  // once; per angle the thread issues all its loads before it stores any.
-  int soff[NS], spk[NS];
-#pragma unroll
-  for (int u = 0; u < NS; ++u) {
-    const int sl = (int)threadIdx.x + 256 * u;
-    const int j = sl / (W * W), k = (sl / W) % W, i = sl % W;
-    soff[u] = -j * nz * nx + k * nx + i;
-    spk[u] = sl < W * W * W ? i | (j << 8) | (k << 16) : 0x7f7f7f;
-  }
+  // The window as a W x W x W box in LDS, slot j * PP + k * RP + i =
+  // sample (lo0 + i, lo1 + j, lo2 + k).  Per angle only the used extent
+  // n0 x n1 x n2 is copied, enumerated densely (i fastest, so consecutive
+  // threads read consecutive samples): element e has i = e mod n0,
+  // k = (e / n0) mod n2, j = e / (n0 n2), the divisions in float32 from
+  // e + 1/2 (exact for the extents here: the product stays far from the
+  // integers).  A thread issues all its loads before it stores any; slots
+  // outside the extent keep older values that no tap reads.
   for (int a = 0; a < nb; ++a) {
     const int4 e = win[(size_t)a * nt + ti];
     const int lo0 = e.x, lo1 = e.y, lo2 = e.z;
     const int n0 = e.w & 0xff, n1 = (e.w >> 8) & 0xff, n2 = (e.w >> 16) & 0xff;
-    const V *base = coef + (size_t)a * vol;
-    const V *org = base + ((size_t)(ny - 1 - lo1) * nz + lo2) * nx + lo0;
+    const bool inner = (e.w >> 24) & 1;
+    const V *org = coef + (size_t)a * vol + ((size_t)(ny - 1 - lo1) * nz + lo2) * nx + lo0;
+    const int total = n0 * n1 * n2;
+    const float inv0 = 1.0f / (float)(n0 > 0 ? n0 : 1),
+                inv2 = 1.0f / (float)(n2 > 0 ? n2 : 1);
     V vals[NS];
+    int dst[NS];
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
-      const int i = spk[u] & 0xff, j = (spk[u] >> 8) & 0xff, k = spk[u] >> 16;
-      const bool use = i < n0 && j < n1 && k < n2;
-      const V *src = use ? org + soff[u] : base;  // unused slots read sample 0
-      vals[u] = *src;
+      const int el = (int)threadIdx.x + 256 * u;
+      dst[u] = -1;
+      if (el < total) {
+        const int r = (int)(((float)el + 0.5f) * inv0);
+        const int i = el - r * n0;
+        const int j = (int)(((float)r + 0.5f) * inv2);
+        const int k = r - j * n2;
+        vals[u] = org[(-j * nz + k) * nx + i];
+        dst[u] = j * PP + k * RP + i;
+      }
     }
     __syncthreads();  // the previous angle's gathers are done
 #pragma unroll
-    for (int u = 0; u < NS; ++u) {
-      const int sl = (int)threadIdx.x + 256 * u;
-      if (sl < W * W * W) buf[sl] = vals[u];
-    }
+    for (int u = 0; u < NS; ++u)
+      if (dst[u] >= 0) buf[dst[u]] = vals[u];
     __syncthreads();
     const double *p = params + 12 * a;
 #pragma unroll
@@ -1832,28 +1859,49 @@ __global__ void __launch_bounds__(256)
       spline_weights<ORDER, R>(c2, w2);
       const int s0 = spline_start<ORDER>(c0), s1 = spline_start<ORDER>(c1),
                 s2 = spline_start<ORDER>(c2);
-#pragma unroll
-      for (int j = 0; j <= ORDER; ++j) {
-        i0[j] = min(max(mirror1(s0 + j, nx) - lo0, 0), W - 1);
-        i1[j] = min(max(mirror1(s1 + j, ny) - lo1, 0), W - 1) * W * W;
-        i2[j] = min(max(mirror1(s2 + j, nz) - lo2, 0), W - 1) * W;
-      }
       V v;
       acc_zero(v);
+      if (inner) {
+        // no tap leaves the volume: the taps sit at constant offsets from
+        // the first one
+        const V *b0 = buf + (s1 - lo1) * PP + (s2 - lo2) * RP + (s0 - lo0);
 #pragma unroll
-      for (int q = 0; q <= ORDER; ++q) {
-        V tq;
-        acc_zero(tq);
+        for (int q = 0; q <= ORDER; ++q) {
+          V tq;
+          acc_zero(tq);
 #pragma unroll
-        for (int r = 0; r <= ORDER; ++r) {
-          const V *row = buf + i1[q] + i2[r];
-          V t;
-          acc_zero(t);
+          for (int r = 0; r <= ORDER; ++r) {
+            const V *row = b0 + q * PP + r * RP;
+            V t;
+            acc_zero(t);
 #pragma unroll
-          for (int pp = 0; pp <= ORDER; ++pp) t = fma_w(w0[pp], row[i0[pp]], t);
-          tq = fma_w(w2[r], t, tq);
+            for (int pp = 0; pp <= ORDER; ++pp) t = fma_w(w0[pp], row[pp], t);
+            tq = fma_w(w2[r], t, tq);
+          }
+          v = fma_w(w1[q], tq, v);
         }
-        v = fma_w(w1[q], tq, v);
+      } else {
+#pragma unroll
+        for (int j = 0; j <= ORDER; ++j) {
+          i0[j] = min(max(mirror1(s0 + j, nx) - lo0, 0), W - 1);
+          i1[j] = min(max(mirror1(s1 + j, ny) - lo1, 0), W - 1) * PP;
+          i2[j] = min(max(mirror1(s2 + j, nz) - lo2, 0), W - 1) * RP;
+        }
+#pragma unroll
+        for (int q = 0; q <= ORDER; ++q) {
+          V tq;
+          acc_zero(tq);
+#pragma unroll
+          for (int r = 0; r <= ORDER; ++r) {
+            const V *row = buf + i1[q] + i2[r];
+            V t;
+            acc_zero(t);
+#pragma unroll
+            for (int pp = 0; pp <= ORDER; ++pp) t = fma_w(w0[pp], row[i0[pp]], t);
+            tq = fma_w(w2[r], t, tq);
+          }
+          v = fma_w(w1[q], tq, v);
+        }
       }
       sum[k] = sum[k] + widen(v);
     }
