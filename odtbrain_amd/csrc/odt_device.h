@@ -402,6 +402,19 @@ __device__ __forceinline__ void spline_weights_t(W t, W *w) {
   }
 }
 
+// First tap of the spline at coordinate c (float64) and its weights in R:
+// float32 weights come from the exact float64 fractional offset, float64
+// weights from spline_weights.
+template <int ORDER, typename R>
+__device__ __forceinline__ int spline_taps(double c, R *w) {
+  const double f = (ORDER & 1) ? floor(c) : floor(c + 0.5);
+  if constexpr (sizeof(R) == 4)
+    spline_weights_t<ORDER, R>((R)(c - f), w);
+  else
+    spline_weights<ORDER, R>(c, w);
+  return (int)f - ORDER / 2;
+}
+
 // Input coordinate of an affine map with scipy.ndimage's operation order:
 // ((0 + o0*m0) + o1*m1 [+ o2*m2]) + off, no fused multiply-add.
 __device__ __forceinline__ double affine2(double o0, double o1, double m0,

@@ -1778,7 +1778,7 @@ __global__ void __launch_bounds__(256)
   constexpr int W = aff_win<ORDER>();
   constexpr int RP = aff_row_pitch<ORDER, V>(), PP = aff_plane_pitch<ORDER, V>();
   constexpr int NS = (W * W * W + 255) / 256;  // window samples per thread
-  __shared__ V buf[W * PP];
+  __shared__ V buf[W * PP + 1];  // + a sink slot for idle lanes
   const int nt = ntx * nty * ntz;
   int ti = blockIdx.x;
   if (xcd_map) {
@@ -1808,9 +1808,12 @@ __global__ void __launch_bounds__(256)
   // sample (lo0 + i, lo1 + j, lo2 + k).  Per angle only the used extent
   // n0 x n1 x n2 is copied, enumerated densely (i fastest, so consecutive
   // threads read consecutive samples): element e has i = e mod n0,
-  // k = (e / n0) mod n2, j = e / (n0 n2), the divisions in float32 from
-  // e + 1/2 (exact for the extents here: the product stays far from the
-  // integers).  A thread issues all its loads before it stores any; slots
+  // k = (e / n0) mod n2, j = e / (n0 n2), the divisions as products with
+  // float32 reciprocals from e + 1/2 (exact for the extents here: the
+  // product stays far from the integers, rcp's error included).  Rounds of
+  // 256 elements run while any element is left (a block-uniform test); in
+  // the last round the idle lanes load the last element and store it to the
+  // sink slot.  A thread issues all its loads before it stores any; slots
   // outside the extent keep older values that no tap reads.
   for (int a = 0; a < nb; ++a) {
     const int4 e = win[(size_t)a * nt + ti];
@@ -1819,27 +1822,28 @@ __global__ void __launch_bounds__(256)
     const bool inner = (e.w >> 24) & 1;
     const V *org = coef + (size_t)a * vol + ((size_t)(ny - 1 - lo1) * nz + lo2) * nx + lo0;
     const int total = n0 * n1 * n2;
-    const float inv0 = 1.0f / (float)(n0 > 0 ? n0 : 1),
-                inv2 = 1.0f / (float)(n2 > 0 ? n2 : 1);
+    const float inv0 = __builtin_amdgcn_rcpf((float)(n0 > 0 ? n0 : 1)),
+                inv2 = __builtin_amdgcn_rcpf((float)(n2 > 0 ? n2 : 1));
     V vals[NS];
     int dst[NS];
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
-      const int el = (int)threadIdx.x + 256 * u;
-      dst[u] = -1;
-      if (el < total) {
+      if (256 * u < total) {
+        const int e0 = (int)threadIdx.x + 256 * u;
+        const bool in = e0 < total;
+        const int el = in ? e0 : total - 1;
         const int r = (int)(((float)el + 0.5f) * inv0);
-        const int i = el - r * n0;
+        const int i = el - __mul24(r, n0);
         const int j = (int)(((float)r + 0.5f) * inv2);
-        const int k = r - j * n2;
-        vals[u] = org[(-j * nz + k) * nx + i];
-        dst[u] = j * PP + k * RP + i;
+        const int k = r - __mul24(j, n2);
+        vals[u] = org[(k - j * nz) * nx + i];
+        dst[u] = in ? __mul24(j, PP) + __mul24(k, RP) + i : W * PP;
       }
     }
     __syncthreads();  // the previous angle's gathers are done
 #pragma unroll
     for (int u = 0; u < NS; ++u)
-      if (dst[u] >= 0) buf[dst[u]] = vals[u];
+      if (256 * u < total) buf[dst[u]] = vals[u];
     __syncthreads();
     const double *p = params + 12 * a;
 #pragma unroll
@@ -1854,11 +1858,8 @@ __global__ void __launch_bounds__(256)
       if (!valid) continue;
       R w0[ORDER + 1], w1[ORDER + 1], w2[ORDER + 1];
       int i0[ORDER + 1], i1[ORDER + 1], i2[ORDER + 1];
-      spline_weights<ORDER, R>(c0, w0);
-      spline_weights<ORDER, R>(c1, w1);
-      spline_weights<ORDER, R>(c2, w2);
-      const int s0 = spline_start<ORDER>(c0), s1 = spline_start<ORDER>(c1),
-                s2 = spline_start<ORDER>(c2);
+      const int s0 = spline_taps<ORDER, R>(c0, w0), s1 = spline_taps<ORDER, R>(c1, w1),
+                s2 = spline_taps<ORDER, R>(c2, w2);
       V v;
       acc_zero(v);
       if (inner) {
