@@ -1822,8 +1822,8 @@ __global__ void __launch_bounds__(256)
     const bool inner = (e.w >> 24) & 1;
     const V *org = coef + (size_t)a * vol + ((size_t)(ny - 1 - lo1) * nz + lo2) * nx + lo0;
     const int total = n0 * n1 * n2;
-    const float inv0 = __builtin_amdgcn_rcpf((float)(n0 > 0 ? n0 : 1)),
-                inv2 = __builtin_amdgcn_rcpf((float)(n2 > 0 ? n2 : 1));
+    if (total == 0) continue;  // no voxel of the tile in range (block-uniform)
+    const float inv0 = __builtin_amdgcn_rcpf((float)n0), inv2 = __builtin_amdgcn_rcpf((float)n2);
     V vals[NS];
     int dst[NS];
 #pragma unroll
