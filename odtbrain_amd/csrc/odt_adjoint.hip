@@ -498,6 +498,210 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// The same transposed gather from a block table of forward taps.  The
+// gradient comes transposed to planes-innermost, gt[z][x][y], so the four
+// lanes that share a coefficient (each taking YC/4 consecutive planes) read
+// one 128-byte line per candidate voxel: a wave's load touches one line per
+// coefficient instead of one per lane and plane.  Per block (a 16 x 16
+// coefficient tile, YC planes, one angle) the voxels of the tile's
+// candidate box get their forward tap data once -- tap start (packed) and
+// the per-axis weights, formed exactly as the forward kernel forms them --
+// and the candidate search reads them from LDS.
+// ---------------------------------------------------------------------------
+template <typename R, int ORDER>
+struct AdjBox {
+  static constexpr int K = ORDER / 2 + 1;  // taps reach at most K past an edge
+  // box side: the candidate ranges of a tile's (virtual) coefficient
+  // rectangle, at most 15 + 2K wide, rotated (factor <= sqrt 2), plus the
+  // candidate square (2h = ORDER + 1, also rotated) and the margins
+  static constexpr int S = (int)(1.41422 * (15 + 2 * K + ORDER + 1)) + 5;
+  static constexpr int N = S * S;
+  static constexpr int INVALID = 0x7fff7fff;  // start 32767: no coefficient matches
+  static constexpr size_t LDS = (size_t)N * (sizeof(int) + 2 * (ORDER + 1) * sizeof(R));
+  // block tables up to 100 KiB (float64 orders 4 and 5 keep k_adj_rotate)
+  static constexpr bool FITS = LDS <= 100 * 1024;
+};
+
+// Forward tap data of output voxel (z, x) (k_rotate_lds2's prep): packed
+// start (z in the low, x in the high 16 bits; INVALID outside the plane) and
+// the weights of taps 0 .. ORDER along each axis.
+template <typename R, int ORDER>
+__device__ __forceinline__ int adj_voxel_taps(int z, int x, double m00, double m01,
+                                              double m10, double m11, double oz,
+                                              double ox, int nz, int nx, R *wz, R *wx) {
+  const double cz = affine2((double)z, (double)x, m00, m01, oz);
+  const double cx = affine2((double)z, (double)x, m10, m11, ox);
+  const bool valid = cz >= 0.0 && cz <= (double)(nz - 1) && cx >= 0.0 &&
+                     cx <= (double)(nx - 1);
+  const double fz = (ORDER & 1) ? floor(cz) : floor(cz + 0.5);
+  const double fx = (ORDER & 1) ? floor(cx) : floor(cx + 0.5);
+  spline_weights_t<ORDER, R>((R)(cz - fz), wz);
+  spline_weights_t<ORDER, R>((R)(cx - fx), wx);
+  const int sz = (int)fz - ORDER / 2, sx = (int)fx - ORDER / 2;
+  return valid ? ((sz & 0xffff) | (sx << 16)) : AdjBox<R, ORDER>::INVALID;
+}
+
+template <typename R, int ORDER, int YC>
+__global__ void __launch_bounds__(256)
+    k_adj_rotate_t(const cpx<R> *__restrict__ gt, const double *__restrict__ params,
+                   cpx<R> *__restrict__ slab, int ny, int nz, int nx) {
+  using Box = AdjBox<R, ORDER>;
+  constexpr int NW = ORDER + 1, K = Box::K;
+  constexpr int PPL = YC / 4;  // planes per lane, 32 bytes
+  static_assert(PPL * sizeof(cpx<R>) == 32, "four lanes per 128-byte line");
+  __shared__ int bs[Box::N];
+  __shared__ R bwz[NW][Box::N];
+  __shared__ R bwx[NW][Box::N];
+  const int ntx = (nx + 15) >> 4;
+  const int tz = blockIdx.x / ntx, tx = blockIdx.x - tz * ntx;
+  const int a = blockIdx.y, y0 = blockIdx.z * YC;
+  const double *p = params + 6 * (size_t)a;
+  const double m00 = p[0], m01 = p[1], m10 = p[2], m11 = p[3];
+  const double oz = p[4], ox = p[5];
+  constexpr double h = 0.5 * (ORDER + 1);
+  const double hz = h * (fabs(m00) + fabs(m10)) + 1e-7;
+  const double hx = h * (fabs(m01) + fabs(m11)) + 1e-7;
+  const int nzc = (int)floor(2.0 * hz) + 1, nxc = (int)floor(2.0 * hx) + 1;
+  // the tile's coefficient rectangle, mirrored virtual positions included
+  const int tz0 = tz * 16, tz1 = min(tz0 + 15, nz - 1);
+  const int tx0 = tx * 16, tx1 = min(tx0 + 15, nx - 1);
+  const int Z0 = tz0 <= K ? -K : tz0, Z1 = tz1 >= nz - 1 - K ? nz - 1 + K : tz1;
+  const int X0 = tx0 <= K ? -K : tx0, X1 = tx1 >= nx - 1 - K ? nx - 1 + K : tx1;
+  // candidate box: the candidate ranges are monotone in the coefficient
+  // position, so the rectangle's corners bound them (one sample of margin)
+  int bz0 = 0, bz1 = 0, bx0 = 0, bx1 = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double dz = (double)((c & 1) ? Z1 : Z0) - oz;
+    const double dx = (double)((c & 2) ? X1 : X0) - ox;
+    const int zl = (int)ceil(m00 * dz + m10 * dx - hz);
+    const int xl = (int)ceil(m01 * dz + m11 * dx - hx);
+    bz0 = c ? min(bz0, zl) : zl;
+    bz1 = c ? max(bz1, zl + nzc - 1) : zl + nzc - 1;
+    bx0 = c ? min(bx0, xl) : xl;
+    bx1 = c ? max(bx1, xl + nxc - 1) : xl + nxc - 1;
+  }
+  bz0 = max(bz0 - 1, 0);
+  bz1 = min(bz1 + 1, nz - 1);
+  bx0 = max(bx0 - 1, 0);
+  bx1 = min(bx1 + 1, nx - 1);
+  const int bh = bz1 - bz0 + 1, bw = bx1 - bx0 + 1;
+  const int nbox = bh > 0 && bw > 0 ? bh * bw : 0;
+  const bool table = nbox <= Box::N;  // else every candidate computes its taps
+  if (table)
+    for (int e = threadIdx.x; e < nbox; e += 256) {
+      const int zz = bz0 + e / bw, xx = bx0 + e % bw;
+      R wz[NW], wx[NW];
+      bs[e] = adj_voxel_taps<R, ORDER>(zz, xx, m00, m01, m10, m11, oz, ox, nz, nx, wz, wx);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        bwz[j][e] = wz[j];
+        bwx[j][e] = wx[j];
+      }
+    }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int yb = y0 + (lane & 3) * PPL;  // this lane's first plane
+  constexpr int NONE = -(1 << 30);
+  for (int it = 0; it < 4; ++it) {
+    const int jz = tz0 + it * 4 + wave, jx = tx0 + (lane >> 2);
+    if (jz > tz1 || jx > tx1) continue;
+    const int vz1 = (jz >= 1 && jz <= K)                    ? -jz
+                    : (jz <= nz - 2 && jz >= nz - 1 - K) ? 2 * nz - 2 - jz
+                                                          : NONE;
+    const int vx1 = (jx >= 1 && jx <= K)                    ? -jx
+                    : (jx <= nx - 2 && jx >= nx - 1 - K) ? 2 * nx - 2 - jx
+                                                          : NONE;
+    cpx<R> acc[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) acc[k] = mk<R>(0, 0);
+    const bool lane_on = yb < ny;  // planes beyond ny: nothing to read
+    for (int iv = 0; iv < 4 && nbox > 0; ++iv) {
+      const int vz = (iv & 1) ? vz1 : jz, vx = (iv & 2) ? vx1 : jx;
+      if (vz == NONE || vx == NONE) continue;
+      const double dz = (double)vz - oz, dx = (double)vx - ox;
+      const int zlo = (int)ceil(m00 * dz + m10 * dx - hz);
+      const int xlo = (int)ceil(m01 * dz + m11 * dx - hx);
+      // this lane's PPL planes of voxel (z, x): 32 bytes (ny is a multiple
+      // of 4)
+      auto add = [&](int z, int x, R w) {
+        const float4 *src =
+            reinterpret_cast<const float4 *>(gt + ((size_t)z * nx + x) * ny + yb);
+        const float4 q0 = src[0], q1 = src[1];
+        cpx<R> v[PPL];
+        static_assert(sizeof(v) == 2 * sizeof(float4), "32-byte lane block");
+        __builtin_memcpy(&v[0], &q0, sizeof(float4));
+        __builtin_memcpy(reinterpret_cast<char *>(v) + sizeof(float4), &q1, sizeof(float4));
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) acc[k] = acc[k] + w * v[k];
+      };
+      if (table) {
+        // candidates in the volume lie in the box (the box is clipped to
+        // the volume): one table read per candidate, one branch for the
+        // accepted ones
+        for (int iz = 0; iz < nzc; ++iz) {
+          const int ez = zlo + iz - bz0;
+          for (int ix = 0; ix < nxc; ++ix) {
+            const int ex = xlo + ix - bx0;
+            const bool inb = (unsigned)ez < (unsigned)bh && (unsigned)ex < (unsigned)bw;
+            const int e = inb ? ez * bw + ex : 0;
+            const int pk = bs[e];
+            const int tzi = vz - (int)(short)(pk & 0xffff), txi = vx - (pk >> 16);
+            const bool ok = inb && lane_on && (unsigned)tzi <= (unsigned)ORDER &&
+                            (unsigned)txi <= (unsigned)ORDER;
+            if (ok) add(bz0 + ez, bx0 + ex, bwz[tzi][e] * bwx[txi][e]);
+          }
+        }
+      } else {
+        for (int iz = 0; iz < nzc; ++iz) {
+          const int z = zlo + iz;
+          if (z < 0 || z >= nz) continue;
+          for (int ix = 0; ix < nxc; ++ix) {
+            const int x = xlo + ix;
+            if (x < 0 || x >= nx) continue;
+            R wz[NW], wx[NW];
+            const int pk = adj_voxel_taps<R, ORDER>(z, x, m00, m01, m10, m11, oz, ox, nz,
+                                                    nx, wz, wx);
+            const int tzi = vz - (int)(short)(pk & 0xffff), txi = vx - (pk >> 16);
+            if (!lane_on || (unsigned)tzi > (unsigned)ORDER || (unsigned)txi > (unsigned)ORDER)
+              continue;
+            R wzv = R(0), wxv = R(0);
+#pragma unroll
+            for (int j = 0; j < NW; ++j) {
+              wzv = j == tzi ? wz[j] : wzv;
+              wxv = j == txi ? wx[j] : wxv;
+            }
+            add(z, x, wzv * wxv);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PPL; ++k)
+      if (yb + k < ny)
+        slab[(((size_t)a * ny + yb + k) * nz + jz) * nx + jx] = acc[k];
+  }
+}
+
+// g[z][y][x] -> gt[z][x][y], 32 x 32 tiles through LDS.
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_adj_transpose(const cpx<R> *__restrict__ g, cpx<R> *__restrict__ gt, int ny, int nx) {
+  __shared__ cpx<R> t[32][33];
+  const int z = blockIdx.z, by = blockIdx.y * 32, bx = blockIdx.x * 32;
+  const int c = threadIdx.x & 31, r0 = threadIdx.x >> 5;
+  for (int r = r0; r < 32; r += 8) {
+    const int y = by + r, x = bx + c;
+    if (y < ny && x < nx) t[r][c] = g[((size_t)z * ny + y) * nx + x];
+  }
+  __syncthreads();
+  for (int r = r0; r < 32; r += 8) {
+    const int x = bx + r, y = by + c;
+    if (y < ny && x < nx) gt[((size_t)z * nx + x) * ny + y] = t[c][r];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename R, int ORDER>
@@ -522,6 +726,53 @@ hipError_t launch_adj_rotate(int order, const cpx<R> *g, int nb,
     case 5: return adj_rotate_launch<R, 5>(g, nb, params, slab, ny, nz, nx, st);
   }
   return hipErrorInvalidValue;
+}
+
+template <typename R, int ORDER>
+static hipError_t adj_rotate_t_launch(const cpx<R> *gt, int nb, const double *params,
+                                      cpx<R> *slab, int ny, int nz, int nx,
+                                      hipStream_t st) {
+  if constexpr (AdjBox<R, ORDER>::FITS) {
+    constexpr int YC = 32 / (int)sizeof(cpx<R>) * 4;
+    dim3 grid(((nz + 15) / 16) * ((nx + 15) / 16), nb, (ny + YC - 1) / YC);
+    k_adj_rotate_t<R, ORDER, YC><<<grid, 256, 0, st>>>(gt, params, slab, ny, nz, nx);
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
+template <typename R>
+bool adj_rotate_t_supported(int order, int ny, int nz, int nx) {
+  const bool fits = order == 2   ? AdjBox<R, 2>::FITS
+                    : order == 3 ? AdjBox<R, 3>::FITS
+                    : order == 4 ? AdjBox<R, 4>::FITS
+                    : order == 5 ? AdjBox<R, 5>::FITS
+                                 : false;
+  return fits && nz >= 8 && nx >= 8 && ny % 4 == 0 && nz <= 32000 && nx <= 32000 &&
+         nz <= 65535;
+}
+
+template <typename R>
+hipError_t launch_adj_rotate_t(int order, const cpx<R> *gt, int nb, const double *params,
+                               cpx<R> *slab, int ny, int nz, int nx, hipStream_t st) {
+  if (!adj_rotate_t_supported<R>(order, ny, nz, nx)) return hipErrorInvalidValue;
+  switch (order) {
+    case 2: return adj_rotate_t_launch<R, 2>(gt, nb, params, slab, ny, nz, nx, st);
+    case 3: return adj_rotate_t_launch<R, 3>(gt, nb, params, slab, ny, nz, nx, st);
+    case 4: return adj_rotate_t_launch<R, 4>(gt, nb, params, slab, ny, nz, nx, st);
+    case 5: return adj_rotate_t_launch<R, 5>(gt, nb, params, slab, ny, nz, nx, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_adj_transpose(const cpx<R> *g, cpx<R> *gt, int ny, int nz, int nx,
+                                hipStream_t st) {
+  if (nz > 65535) return hipErrorInvalidValue;
+  dim3 grid((nx + 31) / 32, (ny + 31) / 32, nz);
+  k_adj_transpose<R><<<grid, 256, 0, st>>>(g, gt, ny, nx);
+  return hipGetLastError();
 }
 
 template <typename R>
@@ -643,6 +894,12 @@ hipError_t launch_adj_fft(int stage, int n, const void *in, void *out,
   template hipError_t launch_adj_rotate<R>(int, const cpx<R> *, int,            \
                                            const double *, cpx<R> *, int, int,  \
                                            int, hipStream_t);                   \
+  template bool adj_rotate_t_supported<R>(int, int, int, int);                  \
+  template hipError_t launch_adj_rotate_t<R>(int, const cpx<R> *, int,          \
+                                             const double *, cpx<R> *, int, int, \
+                                             int, hipStream_t);                  \
+  template hipError_t launch_adj_transpose<R>(const cpx<R> *, cpx<R> *, int, int, \
+                                              int, hipStream_t);                  \
   template hipError_t launch_scale_ends<R>(R *, long long, int, long long,      \
                                            long long, long long, long long,     \
                                            double, hipStream_t);                \

@@ -168,6 +168,16 @@ template <typename R>
 hipError_t launch_adj_rotate(int order, const cpx<R> *g, int nb,
                              const double *params, cpx<R> *slab, int ny, int nz,
                              int nx, hipStream_t st);
+// R_a^T from the planes-innermost gradient gt[z][x][y] (ny a multiple of 4).
+template <typename R>
+bool adj_rotate_t_supported(int order, int ny, int nz, int nx);
+template <typename R>
+hipError_t launch_adj_rotate_t(int order, const cpx<R> *gt, int nb, const double *params,
+                               cpx<R> *slab, int ny, int nz, int nx, hipStream_t st);
+// g[z][y][x] -> gt[z][x][y]
+template <typename R>
+hipError_t launch_adj_transpose(const cpx<R> *g, cpx<R> *gt, int ny, int nz, int nx,
+                                hipStream_t st);
 template <typename R>
 hipError_t launch_scale_ends(R *data, long long nlines, int n, long long n0,
                              long long s0, long long s1, long long es, double s,

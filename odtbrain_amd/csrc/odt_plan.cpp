@@ -193,6 +193,7 @@ struct odt_plan {
   int nb_fr, nb_fc, nb_col, nb_row;  // transforms per block
   size_t esz_c, esz_v;        // bytes of complex element / slab element
   size_t off_F1, off_S, off_T, off_slab, off_comp, off_win, off_part, ws_bytes;
+  size_t off_gt = 0;  // adjoint: the output gradient, planes innermost (or 0)
   // Column and row stages of consecutive chunks overlap: the row stage of
   // chunk c runs on a second stream while the column stage of chunk c + 1
   // fills the other of two T buffers (off_T2).  The column stage is bound
@@ -584,15 +585,23 @@ int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
   // prefilter runs as two out-of-place convolutions (z into the second
   // half, x back), else as the in-place recursions.
   const bool fused = !p->explicit_iir;
+  // The gather reads the gradient transposed to planes-innermost
+  // (k_adj_rotate_t) when the plan has room for the copy, else as it is
+  // (k_adj_rotate).
+  const bool tr = p->off_gt != 0 && !std::getenv("ODTBRAIN_AMD_ADJ_DIRECT");
   const bool fir = !fused && p->NB >= 2;
   const int step = fir ? p->NB / 2 : p->NB;
   cpx<R> *slabB = slab + (size_t)step * per_angle;
+  cpx<R> *gt = tr ? (cpx<R> *)(ws + p->off_gt) : nullptr;
   if (fir && !p->fir_taps)
     return fail(ODT_ERR_UNSUPPORTED, "adjoint: the plan has no prefilter taps");
+  if (tr) HIPCHK(odt::launch_adj_transpose<R>(gout, gt, g.Ny, g.ln, g.Nx, st), "k_adj_transpose");
   for (int b0 = 0; b0 < p->A; b0 += step) {
     const int nb = std::min(step, p->A - b0);
-    HIPCHK(odt::launch_adj_rotate<R>(p->order, gout, nb, p->params + 6 * (size_t)b0,
-                                     slab, g.Ny, g.ln, g.Nx, st),
+    const double *prm = p->params + 6 * (size_t)b0;
+    HIPCHK(tr ? odt::launch_adj_rotate_t<R>(p->order, gt, nb, prm, slab, g.Ny, g.ln, g.Nx, st)
+              : odt::launch_adj_rotate<R>(p->order, gout, nb, prm, slab, g.Ny, g.ln, g.Nx,
+                                          st),
            "k_adj_rotate");
     if (fused) {
       // nothing: the transforms below carry the transposed prefilter
@@ -872,6 +881,16 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
       p->off_part = off;
       off = align256(off + 2 * (size_t)gg * slab_angle);
     }
+  }
+  // the adjoint's copy of the output gradient with the planes innermost
+  // (one volume; plans whose adjoint can use it)
+  p->off_gt = 0;
+  if (d->kind == ODT_KIND_3D && !p->onlyreal && p->order >= 2 &&
+      (d->precision == ODT_FP64
+           ? odt::adj_rotate_t_supported<double>(p->order, g.Ny, g.ln, g.Nx)
+           : odt::adj_rotate_t_supported<float>(p->order, g.Ny, g.ln, g.Nx))) {
+    p->off_gt = off;
+    off = align256(off + slab_angle);
   }
   p->ws_bytes = off;
 
