@@ -799,7 +799,8 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
   }
   if (tenv) t_budget = (size_t)std::atoll(tenv);
   if (tu && tu->t_budget_bytes > 0) t_budget = (size_t)tu->t_budget_bytes;
-  p->NC = (int)std::max<size_t>(1, std::min<size_t>(t_budget / t_angle, (size_t)p->NB));
+  const size_t nc_budget = std::max<size_t>(1, t_budget / t_angle);
+  p->NC = (int)std::min<size_t>(nc_budget, (size_t)p->NB);
   p->NC = std::min(p->NC, 64);
   // float64 forward stage where its transforms fit the LDS in float64
   p->fwd64 = d->precision == ODT_FP64 ||
@@ -815,12 +816,14 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
               (d->precision == ODT_FP64 ? odt::row_fuses_2d<double>(g, true)
                                         : odt::row_fuses_2d<float>(g, true)) &&
               !std::getenv("ODTBRAIN_AMD_NO_FUSE_2D");
-  // When a batch takes two chunks or more, the T budget holds two buffers
-  // of half the chunk each and the column and row stages of consecutive
-  // chunks overlap (256^2 x 360 complex64: 57.8 -> 56.1 ms per
-  // reconstruction with 30-angle chunks).
+  // When a batch takes two chunks or more, the column and row stages of
+  // consecutive chunks overlap through two T buffers, which share the T
+  // budget when it binds (256^2 x 360 complex64: 57.8 -> 56.1 ms per
+  // reconstruction with 34-angle chunks).  Fused 2D plans have no T: one
+  // chunk per slab batch (F1 and S are a row per angle).
+  if (p->fuse2d) p->NC = p->NB;
   p->overlap = !p->fuse2d && p->NC < p->NB && !std::getenv("ODTBRAIN_AMD_NO_OVERLAP");
-  if (p->overlap) p->NC = std::max(1, (p->NC + 1) / 2);
+  if (p->overlap) p->NC = std::max(1, std::min(p->NC, (int)std::min<size_t>(nc_budget / 2, 64)));
 
   size_t off = 0;
   p->off_F1 = off;
