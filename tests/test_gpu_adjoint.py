@@ -169,3 +169,35 @@ def test_adjoint_slab_budgets_agree(monkeypatch, budget_angles):
     Mu = G.backpropagate_3d(u, angles, 6.0, 1.333, dtype=np.float64)
     lhs, rhs = _dot(g, Mu), _dot(part, u)
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+@pytest.mark.parametrize("dtype,orders,tol", [(np.float64, (2, 3), 1e-13),
+                                              (np.float32, (2, 3, 4, 5), 2e-6)])
+@pytest.mark.parametrize("ny", [20, 18])
+def test_adjoint_transposed_gather_equals_direct(monkeypatch, dtype, orders,
+                                                 tol, ny):
+    """The rotate transpose from the planes-innermost copy of the gradient
+    with a per-block table of forward taps (k_adj_rotate_t) and the direct
+    gather (k_adj_rotate, ODTBRAIN_AMD_ADJ_DIRECT) agree (their spline
+    weights come from two formulas of the same polynomials), and the
+    transposed one passes the dot-product test.  Ny = 18 is not a multiple
+    of 4: both runs take the direct gather."""
+    A, nx = 11, 24
+    rng = np.random.default_rng(21)
+    angles = np.sort(rng.uniform(0, 2 * np.pi, A))
+    cdt = np.complex64 if dtype == np.float32 else np.complex128
+    u = _rand((A, ny, nx), cdt, 22)
+    g = _rand((nx, ny, nx), cdt, 23)
+    for order in orders:
+        kw = dict(dtype=dtype, intp_order=order)
+        t = AG.backpropagate_3d_adjoint(g, angles, 6.0, 1.333, **kw)
+        monkeypatch.setenv("ODTBRAIN_AMD_ADJ_DIRECT", "1")
+        d = AG.backpropagate_3d_adjoint(g, angles, 6.0, 1.333, **kw)
+        monkeypatch.delenv("ODTBRAIN_AMD_ADJ_DIRECT")
+        err = float(torch.linalg.vector_norm((t - d).to(torch.complex128)) /
+                    torch.linalg.vector_norm(d.to(torch.complex128)))
+        assert err < tol, (order, err)
+        Mu = G.backpropagate_3d(u, angles, 6.0, 1.333, **kw)
+        lhs, rhs = _dot(g, Mu), _dot(t, u)
+        bar = 1e-12 if dtype == np.float64 else 1e-5
+        assert abs(lhs - rhs) <= bar * abs(lhs), (order, lhs, rhs)

@@ -391,3 +391,27 @@ def test_2d_angle_groups_across_slab_batches(monkeypatch, batch_angles,
         else:
             assert rel_l2(got, ref) < FP32_REL_L2
             assert rel_l2(got, whole32) < 1e-6
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-7), (np.float64, 1e-14)])
+def test_3d_overlapped_filter_chunks_equal_one_stream(monkeypatch, dtype, tol):
+    """A T budget of a few angles splits the filter stage into chunks; the
+    row transforms of chunk c then run on the plan's second stream while
+    the column transforms of chunk c + 1 fill the other T buffer.  The
+    per-angle arithmetic does not depend on the chunking or the stream, so
+    the result equals the one-stream plan's (ODTBRAIN_AMD_NO_OVERLAP) and
+    the oracle's.  The env variables are read at plan creation: the plan
+    cache is cleared around each variant."""
+    from odtbrain_amd import _core
+    sino, angles = create_test_sino_3d(A=23, Nx=24, Ny=18)
+    kw = dict(res=3.0, nm=1.335, dtype=dtype)
+    monkeypatch.setenv("ODTBRAIN_AMD_T_BUDGET", str(1 << 20))
+    _core._PLAN_CACHE.clear()
+    ov = G.backpropagate_3d(sino, angles, **kw)
+    monkeypatch.setenv("ODTBRAIN_AMD_NO_OVERLAP", "1")
+    _core._PLAN_CACHE.clear()
+    one = G.backpropagate_3d(sino, angles, **kw)
+    _core._PLAN_CACHE.clear()
+    assert rel_l2(ov, one) < tol
+    ref = O.backpropagate_3d(sino, angles, 3.0, 1.335)
+    assert rel_l2(ov, ref) < (FP32_REL_L2 if dtype == np.float32 else 1e-9)
