@@ -645,10 +645,26 @@ __global__ void __launch_bounds__(256)
       }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < ny * kColCpb; idx += blockDim.x) {
-      const int y = idx / kColCpb, cl = idx - y * kColCpb;
-      if (c0 + cl < g.Lx)
-        st_nt(trow + (size_t)y * ldt + c0 + cl, tile[y * (kColCpb + 1) + cl]);
+    {
+      // the tile out in 128-byte row segments: a thread keeps its column
+      // and takes every RPP-th row, four rows' LDS reads before their stores
+      constexpr int NTH = TPB * TT;  // the block's threads
+      constexpr int RPP = NTH / kColCpb;
+      static_assert(NTH % kColCpb == 0, "whole row segments per pass");
+      const int cl = threadIdx.x % kColCpb;
+      if (c0 + cl < g.Lx) {
+        const cpx<R> *src = tile + cl;
+        cpx<R> *dst = trow + c0 + cl;
+        int y = threadIdx.x / kColCpb;
+        for (; y + 3 * RPP < ny; y += 4 * RPP) {
+          cpx<R> v4[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v4[q] = src[(y + q * RPP) * (kColCpb + 1)];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) st_nt(dst + (size_t)(y + q * RPP) * ldt, v4[q]);
+        }
+        for (; y < ny; y += RPP) st_nt(dst + (size_t)y * ldt, src[y * (kColCpb + 1)]);
+      }
     }
     __syncthreads();
   }
