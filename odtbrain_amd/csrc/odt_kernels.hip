@@ -708,6 +708,16 @@ __global__ void __launch_bounds__(512)
   typename FFT::Twid tw;
   FFT::load_twiddles(tw, twx, j);
   const int iters = (rb + tpb - 1) / tpb;
+  // the fused x-prefilter's boundary weights of the lane's output columns
+  // (the same for every row)
+  R qa[8], qb[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int x = j + m * TT - g.padxl;
+    const bool in = g.fuse_x && (uni ? (m >= mlo && m < mhi) : (x >= 0 && x < nx));
+    qa[m] = in ? q0x[x] : R(0);
+    qb[m] = in ? q0x[nx - 1 - x] : R(0);
+  }
   // fused 2D rows: this angle's spectrum values of the lane's slots
   cpx<R> s2[8];
   if (S2) {
@@ -759,7 +769,7 @@ __global__ void __launch_bounds__(512)
         const bool in = uni ? (m >= mlo && m < mhi) : (x >= 0 && x < nx);
         if (in) {
           cpx<R> val = v[m];
-          if (g.fuse_x) val = val + q0x[x] * R0 + q0x[nx - 1 - x] * R1;
+          if (g.fuse_x) val = val + qa[m] * R0 + qb[m] * R1;
           store_val(dst + x, val);
         }
       }
