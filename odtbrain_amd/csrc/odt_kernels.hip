@@ -476,6 +476,16 @@ __global__ void __launch_bounds__(512)
   }
   const bool uni = g.padyl % TT == 0 && ny % TT == 0;
   const int mlo = g.padyl / TT, mhi = (g.padyl + ny) / TT;
+  // the fused y-prefilter's boundary weights of the lane's output rows (the
+  // same for every transform)
+  R wya[8], wyb[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int y = j + m * TT - g.padyl;
+    const bool in = g.fuse_y && (uni ? (m >= mlo && m < mhi) : (y >= 0 && y < ny));
+    wya[m] = in ? q0y[y] : R(0);
+    wyb[m] = in ? q0y[ny - 1 - y] : R(0);
+  }
   typename FFT::Twid tw;
   FFT::load_twiddles(tw, twy, j);
   for (int k = k0; k < kend; ++k) {
@@ -527,7 +537,7 @@ __global__ void __launch_bounds__(512)
         const bool in = uni ? (m >= mlo && m < mhi) : (y >= 0 && y < ny);
         if (live && in) {
           cpx<R> val = v[m];
-          if (g.fuse_y) val = val + q0y[y] * R0 + q0y[ny - 1 - y] * R1;
+          if (g.fuse_y) val = val + wya[m] * R0 + wyb[m] * R1;
           if (staged)
             tile[y * (kColCpb + 1) + cl] = val;
           else
@@ -584,6 +594,16 @@ __global__ void __launch_bounds__(256)
   }
   const bool uni = g.padyl % TT == 0 && ny % TT == 0;
   const int mlo = g.padyl / TT, mhi = (g.padyl + ny) / TT;
+  // the fused y-prefilter's boundary weights of the lane's output rows (the
+  // same for every transform)
+  R wya[8], wyb[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int y = j + m * TT - g.padyl;
+    const bool in = g.fuse_y && (uni ? (m >= mlo && m < mhi) : (y >= 0 && y < ny));
+    wya[m] = in ? q0y[y] : R(0);
+    wyb[m] = in ? q0y[ny - 1 - y] : R(0);
+  }
   typename FFT::Twid tw;
   FFT::load_twiddles(tw, twy, j);
   // this slot's propagator values, all angles
@@ -639,7 +659,7 @@ __global__ void __launch_bounds__(256)
         const bool in = uni ? (m >= mlo && m < mhi) : (y >= 0 && y < ny);
         if (live[i] && in) {
           cpx<R> val = v[m];
-          if (g.fuse_y) val = val + q0y[y] * R0 + q0y[ny - 1 - y] * R1;
+          if (g.fuse_y) val = val + wya[m] * R0 + wyb[m] * R1;
           tile[y * (kColCpb + 1) + cl] = val;
         }
       }
