@@ -132,11 +132,12 @@ int odt_plan_workspace_size(const odt_plan *plan, size_t *bytes);
 /* Run the full reconstruction on `stream` (a hipStream_t, 0 = null stream).
  * d_usin: (A, Ny, Nx) complex64 (FP32) or complex128 (FP64), C order.
  * d_out:  (Nx, Ny, Nx) complex, or real if onlyreal; overwritten.
- * The call is asynchronous with respect to the host.  3D plans whose
- * angles take two filter chunks or more run the row transforms on a second
- * stream owned by the plan (forked from and joined back into `stream` with
- * events, so the work is ordered on `stream` as a whole); under graph
- * capture the call keeps to `stream`. */
+ * The call is asynchronous with respect to the host.  With
+ * ODTBRAIN_AMD_OVERLAP=1 at plan creation, 3D plans whose angles take two
+ * filter chunks or more run the row transforms on a second stream owned by
+ * the plan (forked from and joined back into `stream` with events, so the
+ * work is ordered on `stream` as a whole); under graph capture the call
+ * keeps to `stream`. */
 int odt_plan_execute(odt_plan *plan, const void *d_usin, void *d_out,
                      void *d_workspace, size_t workspace_bytes, void *stream);
 

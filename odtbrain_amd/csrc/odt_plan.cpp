@@ -194,10 +194,9 @@ struct odt_plan {
   size_t esz_c, esz_v;        // bytes of complex element / slab element
   size_t off_F1, off_S, off_T, off_slab, off_comp, off_win, off_part, ws_bytes;
   size_t off_gt = 0;  // adjoint: the output gradient, planes innermost (or 0)
-  // Column and row stages of consecutive chunks overlap: the row stage of
-  // chunk c runs on a second stream while the column stage of chunk c + 1
-  // fills the other of two T buffers (off_T2).  The column stage is bound
-  // by its arithmetic, the row stage by HBM traffic.
+  // Column and row stages of consecutive chunks overlapped (opt-in): the
+  // row stage of chunk c runs on a second stream while the column stage of
+  // chunk c + 1 fills the other of two T buffers (off_T2).
   bool overlap = false;
   size_t off_T2 = 0;
   hipStream_t side = nullptr;
@@ -816,13 +815,16 @@ int odt_plan_create_tuned(const odt_desc *d, const odt_tuning *tu, odt_plan **ou
               (d->precision == ODT_FP64 ? odt::row_fuses_2d<double>(g, true)
                                         : odt::row_fuses_2d<float>(g, true)) &&
               !std::getenv("ODTBRAIN_AMD_NO_FUSE_2D");
-  // When a batch takes two chunks or more, the column and row stages of
-  // consecutive chunks overlap through two T buffers, which share the T
-  // budget when it binds (256^2 x 360 complex64: 57.8 -> 56.1 ms per
-  // reconstruction with 34-angle chunks).  Fused 2D plans have no T: one
-  // chunk per slab batch (F1 and S are a row per angle).
+  // Optionally (ODTBRAIN_AMD_OVERLAP=1), when a batch takes two chunks or
+  // more, the column and row stages of consecutive chunks overlap through
+  // two T buffers, which share the T budget when it binds.  It paid while
+  // the row stage was latency-bound (256^2 x 360 complex64: 57.8 -> 56.1 ms
+  // per reconstruction); with both stages near the copy rate it costs
+  // (48.9 -> 50.0 ms), so one stream is the default.  Fused 2D plans have
+  // no T: one chunk per slab batch (F1 and S are a row per angle).
   if (p->fuse2d) p->NC = p->NB;
-  p->overlap = !p->fuse2d && p->NC < p->NB && !std::getenv("ODTBRAIN_AMD_NO_OVERLAP");
+  const char *ovenv = std::getenv("ODTBRAIN_AMD_OVERLAP");
+  p->overlap = !p->fuse2d && p->NC < p->NB && ovenv && *ovenv && *ovenv != '0';
   if (p->overlap) p->NC = std::max(1, std::min(p->NC, (int)std::min<size_t>(nc_budget / 2, 64)));
 
   size_t off = 0;

@@ -395,20 +395,21 @@ def test_2d_angle_groups_across_slab_batches(monkeypatch, batch_angles,
 
 @pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-7), (np.float64, 1e-14)])
 def test_3d_overlapped_filter_chunks_equal_one_stream(monkeypatch, dtype, tol):
-    """A T budget of a few angles splits the filter stage into chunks; the
-    row transforms of chunk c then run on the plan's second stream while
-    the column transforms of chunk c + 1 fill the other T buffer.  The
-    per-angle arithmetic does not depend on the chunking or the stream, so
-    the result equals the one-stream plan's (ODTBRAIN_AMD_NO_OVERLAP) and
+    """A T budget of a few angles splits the filter stage into chunks; with
+    ODTBRAIN_AMD_OVERLAP=1 the row transforms of chunk c then run on the
+    plan's second stream while the column transforms of chunk c + 1 fill
+    the other T buffer.  The per-angle arithmetic does not depend on the
+    chunking or the stream, so the result equals the one-stream plan's and
     the oracle's.  The env variables are read at plan creation: the plan
     cache is cleared around each variant."""
     from odtbrain_amd import _core
     sino, angles = create_test_sino_3d(A=23, Nx=24, Ny=18)
     kw = dict(res=3.0, nm=1.335, dtype=dtype)
     monkeypatch.setenv("ODTBRAIN_AMD_T_BUDGET", str(1 << 20))
+    monkeypatch.setenv("ODTBRAIN_AMD_OVERLAP", "1")
     _core._PLAN_CACHE.clear()
     ov = G.backpropagate_3d(sino, angles, **kw)
-    monkeypatch.setenv("ODTBRAIN_AMD_NO_OVERLAP", "1")
+    monkeypatch.delenv("ODTBRAIN_AMD_OVERLAP")
     _core._PLAN_CACHE.clear()
     one = G.backpropagate_3d(sino, angles, **kw)
     _core._PLAN_CACHE.clear()
