@@ -437,6 +437,91 @@ __device__ __forceinline__ void wave_boundary_terms(const cpx<R> *v, int p,
 // stage skips the others (half the loads and propagators).
 // Loads are predicated with selects on clamped indices, and when the crop
 // window is aligned to N/8 the crop test is uniform across the wavefront.
+// Forward stage with the register radix-8 engine (power-of-two lengths 64
+// to 2048; longer ones and other lengths keep k_fwd_rows / k_fwd_cols):
+// one padded row (or one support column) per transform, 256 / (N/8)
+// transforms per block.  Rows: the transform's outputs go through its LDS
+// buffer so that the block writes F1's support columns with its rows
+// consecutive, as k_fwd_rows does.  Columns: the outputs of the support
+// bins times the prefactor go straight to S.
+template <typename RI, typename C, int N>
+__global__ void __launch_bounds__(256)
+    k_fwd_rows8(const cpx<RI> *__restrict__ usin, const double *__restrict__ weights,
+                cpx<C> *__restrict__ F1, const cpx<C> *__restrict__ twx, Geom g,
+                double pad_re, double pad_im, int a0) {
+  using FFT = Fft8<C, N, -1>;
+  constexpr int TT = FFT::T;
+  static_assert(TT <= 256, "one block holds a whole transform");
+  constexpr int TPB = 256 / TT;
+  const int t = threadIdx.x / TT, jj = threadIdx.x - t * TT;
+  cpx<C> *lds = reinterpret_cast<cpx<C> *>(odt_smem);
+  cpx<C> *b0 = lds + FFT::NBUF * t * FFT::PADN, *b1 = b0 + FFT::PADN;
+  const int ai = blockIdx.y, a = a0 + ai;
+  const bool edge = g.pad_mode == 1;
+  const int jend = edge ? g.padyl + g.Ny : g.lNy;
+  const int jb = blockIdx.x * TPB + (edge ? g.padyl : 0);
+  const int j = jb + t;
+  const cpx<RI> *u = usin + (size_t)a * g.Ny * g.Nx;
+  const C w = (C)weights[a];
+  const cpx<C> end = mk<C>((C)pad_re, (C)pad_im);
+  cpx<C> v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    v[m] = j < jend ? padded_value<C>(u, g, w, end, j, jj + m * TT) : mk<C>(0, 0);
+  typename FFT::Twid tw;
+  FFT::load_twiddles(tw, twx, jj);
+  FFT::template run<false>(v, b0, b1, tw, jj);
+  __syncthreads();  // every transform's last stage has read its buffers
+#pragma unroll
+  for (int m = 0; m < 8; ++m) b0[lds_pad(jj + m * TT)] = v[m];
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < g.Lx * TPB; idx += 256) {
+    const int c = idx / TPB, r = idx - c * TPB;
+    if (jb + r < jend) {
+      const cpx<C> *res = lds + FFT::NBUF * r * FFT::PADN;
+      F1[((size_t)ai * g.Lx + c) * g.lNy + jb + r] = res[lds_pad(wrap(g.jx0 + c, N))];
+    }
+  }
+}
+
+template <typename RI, typename C, int N>
+__global__ void __launch_bounds__(256)
+    k_fwd_cols8(const cpx<C> *__restrict__ F1, cpx<RI> *__restrict__ S,
+                const cpx<C> *__restrict__ twy, const cpx<C> *__restrict__ P, Geom g) {
+  using FFT = Fft8<C, N, -1>;
+  constexpr int TT = FFT::T;
+  static_assert(TT <= 256, "one block holds a whole transform");
+  constexpr int TPB = 256 / TT;
+  const int t = threadIdx.x / TT, jj = threadIdx.x - t * TT;
+  cpx<C> *lds = reinterpret_cast<cpx<C> *>(odt_smem);
+  cpx<C> *b0 = lds + FFT::NBUF * t * FFT::PADN, *b1 = b0 + FFT::PADN;
+  const int ai = blockIdx.y;
+  const int c = blockIdx.x * TPB + t;
+  const bool live = c < g.Lx;
+  const bool edge = g.pad_mode == 1;
+  const cpx<C> *col = F1 + ((size_t)ai * g.Lx + (live ? c : 0)) * g.lNy;
+  cpx<C> v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int jr0 = jj + m * TT;
+    // edge padding: the padded rows hold the nearest detector row
+    const int jr = edge ? min(max(jr0, g.padyl), g.padyl + g.Ny - 1) : jr0;
+    v[m] = live ? col[jr] : mk<C>(0, 0);
+  }
+  typename FFT::Twid tw;
+  FFT::load_twiddles(tw, twy, jj);
+  FFT::template run<false>(v, b0, b1, tw, jj);
+  if (!live) return;  // (after the transform's barriers)
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int cy = freq_of_bin<N>(jj + m * TT) - g.jy0;
+    if (cy >= 0 && cy < g.Ly) {
+      const cpx<C> x = v[m] * P[(size_t)c * g.Ly + cy];
+      S[((size_t)ai * g.Lx + c) * g.Ly + cy] = mk<RI>((RI)x.re, (RI)x.im);
+    }
+  }
+}
+
 constexpr int kColCpb = 16;
 
 template <int N, bool BAND>
@@ -2159,11 +2244,54 @@ hipError_t launch_dft_table(cpx<R> *tw, int n, cpx<double> *scratch64, hipStream
   return e;
 }
 
+template <typename RI, typename C, int N>
+static hipError_t fwd_rows8_launch(const cpx<RI> *usin, const double *w, cpx<C> *F1,
+                                   const cpx<C> *twx, const Geom &g, double pr, double pi,
+                                   int a0, int na, hipStream_t st) {
+  using FFT = Fft8<C, N, -1>;
+  constexpr int TPB = 256 / FFT::T;
+  const size_t sh = (size_t)TPB * FFT::NBUF * FFT::PADN * sizeof(cpx<C>);
+  hipError_t e = set_lds((const void *)k_fwd_rows8<RI, C, N>, sh);
+  if (e != hipSuccess) return e;
+  const int rows = g.pad_mode == 1 ? g.Ny : g.lNy;  // edge: detector rows only
+  dim3 grid((rows + TPB - 1) / TPB, na);
+  k_fwd_rows8<RI, C, N><<<grid, 256, sh, st>>>(usin, w, F1, twx, g, pr, pi, a0);
+  return hipGetLastError();
+}
+
+template <typename RI, typename C, int N>
+static hipError_t fwd_cols8_launch(const cpx<C> *F1, cpx<RI> *S, const cpx<C> *twy,
+                                   const cpx<C> *P, const Geom &g, int na, hipStream_t st) {
+  using FFT = Fft8<C, N, -1>;
+  constexpr int TPB = 256 / FFT::T;
+  const size_t sh = (size_t)TPB * FFT::NBUF * FFT::PADN * sizeof(cpx<C>);
+  hipError_t e = set_lds((const void *)k_fwd_cols8<RI, C, N>, sh);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.Lx + TPB - 1) / TPB, na);
+  k_fwd_cols8<RI, C, N><<<grid, 256, sh, st>>>(F1, S, twy, P, g);
+  return hipGetLastError();
+}
+
+// ODTBRAIN_AMD_FWD_GENERIC: the forward stage keeps the LDS radix-2 engine
+inline bool fwd_generic() {
+  static const bool f = std::getenv("ODTBRAIN_AMD_FWD_GENERIC") != nullptr;
+  return f;
+}
+
 template <typename RI, typename C>
 hipError_t launch_fwd_rows(const cpx<RI> *usin, const double *w, cpx<C> *F1,
                            const cpx<C> *twx, const Geom &g, double pr,
                            double pi, int a0, int na, int nb,
                            hipStream_t st) {
+  if (!fwd_generic()) switch (g.lNx) {
+      case 64: return fwd_rows8_launch<RI, C, 64>(usin, w, F1, twx, g, pr, pi, a0, na, st);
+      case 128: return fwd_rows8_launch<RI, C, 128>(usin, w, F1, twx, g, pr, pi, a0, na, st);
+      case 256: return fwd_rows8_launch<RI, C, 256>(usin, w, F1, twx, g, pr, pi, a0, na, st);
+      case 512: return fwd_rows8_launch<RI, C, 512>(usin, w, F1, twx, g, pr, pi, a0, na, st);
+      case 1024: return fwd_rows8_launch<RI, C, 1024>(usin, w, F1, twx, g, pr, pi, a0, na, st);
+      case 2048: return fwd_rows8_launch<RI, C, 2048>(usin, w, F1, twx, g, pr, pi, a0, na, st);
+      default: break;
+    }
   const size_t sh = lds_bytes(g.lNx, nb, sizeof(cpx<C>));
   hipError_t e = set_lds((const void *)k_fwd_rows<RI, C>, sh);
   if (e != hipSuccess) return e;
@@ -2177,6 +2305,15 @@ template <typename RI, typename C>
 hipError_t launch_fwd_cols(const cpx<C> *F1, cpx<RI> *S, const cpx<C> *twy,
                            const cpx<C> *P, const Geom &g, int na, int nb,
                            hipStream_t st) {
+  if (!fwd_generic()) switch (g.lNy) {
+      case 64: return fwd_cols8_launch<RI, C, 64>(F1, S, twy, P, g, na, st);
+      case 128: return fwd_cols8_launch<RI, C, 128>(F1, S, twy, P, g, na, st);
+      case 256: return fwd_cols8_launch<RI, C, 256>(F1, S, twy, P, g, na, st);
+      case 512: return fwd_cols8_launch<RI, C, 512>(F1, S, twy, P, g, na, st);
+      case 1024: return fwd_cols8_launch<RI, C, 1024>(F1, S, twy, P, g, na, st);
+      case 2048: return fwd_cols8_launch<RI, C, 2048>(F1, S, twy, P, g, na, st);
+      default: break;
+    }
   const size_t sh = lds_bytes(g.lNy, nb, sizeof(cpx<C>));
   hipError_t e = set_lds((const void *)k_fwd_cols<RI, C>, sh);
   if (e != hipSuccess) return e;
