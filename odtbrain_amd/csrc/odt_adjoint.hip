@@ -29,6 +29,8 @@
 // transform per wavefront or less).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "odt_device.h"
 #include "odt_fft8.h"
 #include "odt_launch.h"
@@ -404,6 +406,112 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int m = 0; m < 8; ++m)
     if (cyv[m]) dst[cyc[m]] = acc[m];
+}
+
+// k_adj_col with the slices' T rows staged through LDS: a block takes CPB
+// consecutive support columns (rows of 64 or 128 bytes, read whole) in NP
+// passes of TPB transforms; slice k + 1's tile is loaded into registers
+// while slice k is transformed.  The sums are the same, in the same order,
+// as k_adj_col's.
+template <int N>
+struct AdjColTile {
+  static constexpr int TT = N / 8;
+  static constexpr int TPB = 256 / TT;
+  static constexpr int CPB0 = TPB > 16 ? TPB : 16;
+  static constexpr int CPB = CPB0 < 2 * TPB ? CPB0 : 2 * TPB;  // columns per block
+  static constexpr int NP = CPB / TPB;                          // passes
+  static constexpr int NE = 12;  // tile elements per thread at most
+};
+
+template <typename R, int N>
+__global__ void __launch_bounds__(256)
+    k_adj_col_t(const cpx<R> *__restrict__ Tp, cpx<R> *__restrict__ Sp,
+                const cpx<R> *__restrict__ twy, const double *__restrict__ prop_a,
+                Geom g, int ldt, const cpx<R> *__restrict__ gt) {
+  using FFT = Fft8<R, N, -1>;
+  using CT = AdjColTile<N>;
+  constexpr int TT = CT::TT, TPB = CT::TPB, CPB = CT::CPB, NP = CT::NP, NE = CT::NE;
+  static_assert(TT <= 64, "one transform per wavefront at most");
+  const int t = threadIdx.x / TT, j = threadIdx.x - t * TT;
+  cpx<R> *lds = reinterpret_cast<cpx<R> *>(odt_smem);
+  cpx<R> *b0 = lds + t * FFT::PADN;
+  cpx<R> *tile = lds + TPB * FFT::PADN;  // ny rows x (CPB + 1)
+  const int c0 = blockIdx.x * CPB, ai = blockIdx.y;
+  const int ln = g.ln, ny = g.Ny;
+  typename FFT::Twid tw;
+  FFT::load_twiddles(tw, twy, j);
+  int cyc[8];
+  bool cyv[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int cy = sfreq<N>(j + m * TT) - g.jy0;
+    cyv[m] = cy >= 0 && cy < g.Ly;
+    cyc[m] = cyv[m] ? cy : 0;
+  }
+  const int ne = ny * CPB;  // <= NE * 256 (the launcher checks)
+  // tile element e = threadIdx.x + 256 u: row e / CPB, column e % CPB
+  cpx<R> pf[NE];
+  auto fetch = [&](int k) {
+    const cpx<R> *src = Tp + (((size_t)ai * ln + k) * ny) * ldt + c0;
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = (int)threadIdx.x + 256 * u;
+      const int r = e / CPB, cl = e - r * CPB;
+      pf[u] = e < ne && c0 + cl < g.Lx ? src[(size_t)r * ldt + cl] : mk<R>(0, 0);
+    }
+  };
+  cpx<R> acc[NP][8];
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[q][m] = mk<R>(0, 0);
+  fetch(0);
+  for (int k = 0; k < ln; ++k) {
+    __syncthreads();  // the previous slice's tile reads are done
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = (int)threadIdx.x + 256 * u;
+      if (e < ne) tile[(e / CPB) * (CPB + 1) + e % CPB] = pf[u];
+    }
+    __syncthreads();
+    if (k + 1 < ln) fetch(k + 1);
+    const double z = g.z0 + (double)k;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const int cl = q * TPB + t, c = c0 + cl;
+      const int cs = c < g.Lx ? c : 0;
+      cpx<R> v[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int y = j + m * TT - g.padyl;
+        v[m] = (y >= 0 && y < ny) ? tile[y * (CPB + 1) + cl] : mk<R>(0, 0);
+      }
+      FFT::run(v, b0, nullptr, tw, j);
+      const cpx<R> *gk = gt ? gt + ((size_t)k * g.Lx + cs) * g.Ly : nullptr;
+      const double *pa = prop_a + (size_t)cs * g.Ly;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        if (!cyv[m]) continue;
+        cpx<R> e;
+        if (gk) {
+          const cpx<R> qv = gk[cyc[m]];
+          e = mk<R>(qv.re, -qv.im);
+        } else {
+          e = phasor(-pa[cyc[m]] * z, R(0));
+        }
+        acc[q][m] = acc[q][m] + v[m] * e;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int c = c0 + q * TPB + t;
+    if (c >= g.Lx) continue;
+    cpx<R> *dst = Sp + ((size_t)ai * g.Lx + c) * g.Ly;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      if (cyv[m]) dst[cyc[m]] = acc[q][m];
+  }
 }
 
 // Sum of the padded samples that np.pad copies to detector sample r
@@ -846,6 +954,23 @@ static hipError_t adj_fft_launch(int stage, const void *in, void *out,
                                            (const cpx<R> *)tw, g, rb, ldt,
                                            (const R *)aux);
   } else if (stage == 1) {
+    using CT = AdjColTile<N>;
+    const size_t sht =
+        ((size_t)CT::TPB * lds_padded(N) + (size_t)g.Ny * (CT::CPB + 1)) * sizeof(cpx<R>);
+    if ((size_t)g.Ny * CT::CPB <= (size_t)CT::NE * 256 && sht <= 160 * 1024 &&
+        !std::getenv("ODTBRAIN_AMD_ADJ_COL_DIRECT")) {
+      if (sht > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_adj_col_t<R, N>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)sht);
+        if (e != hipSuccess) return e;
+      }
+      dim3 grid((g.Lx + CT::CPB - 1) / CT::CPB, na);
+      k_adj_col_t<R, N><<<grid, 256, sht, st>>>((const cpx<R> *)in, (cpx<R> *)out,
+                                                (const cpx<R> *)tw, prop_a, g, ldt,
+                                                (const cpx<R> *)aux);
+      return hipGetLastError();
+    }
     dim3 grid((g.Lx + tpb - 1) / tpb, na);
     k_adj_col<R, N><<<grid, 256, sh, st>>>((const cpx<R> *)in, (cpx<R> *)out,
                                            (const cpx<R> *)tw, prop_a, g, ldt,
