@@ -784,8 +784,16 @@ __global__ void __launch_bounds__(256)
 // S2 non-null (2D, rows viewed as slices): there is no T; row r of angle a is
 // S2[a][c] * gt[r][c], the support spectrum times the plan's propagator
 // table (k_prop_2d's product, formed on load).
+//
+// float32 transforms of 1024 points and more (two LDS buffers, a barrier per
+// stage) are held to 128 VGPRs, four waves per SIMD: two 512-thread blocks
+// per CU instead of one, at the price of a few spilled loop invariants
+// (measured on the batch of mixed 2D/3D reconstructions: 136.8 -> 133.7 ms
+// per replay).  The same cap on the column stage (166 VGPRs) costs more in
+// spills than it gains.
 template <typename R, typename V, int N, bool BAND>
 __global__ void __launch_bounds__(512)
+    __attribute__((amdgpu_waves_per_eu(N >= 1024 && sizeof(R) == 4 ? 4 : 1)))
     k_row_fft8(const cpx<R> *__restrict__ T, V *__restrict__ slab,
                const cpx<R> *__restrict__ twx, const R *__restrict__ q0x,
                Geom g, int rb, int slab0, int ldt, const cpx<R> *__restrict__ S2,
