@@ -12,16 +12,22 @@
 //   k_uw_reliability   wrapped second differences per pixel (borders: 1e7)
 //   k_uw_edges         4-neighbour edges, key = sum of the two reliabilities
 //   segmented radix sort (rocPRIM, stable: equal keys keep creation order)
-//   k_uw_gather        one thread per image merges groups edge by edge in
-//                      sorted order (union by size with whole-turn offsets,
-//                      the published merge rules); a helper wavefront
-//                      warms L2 with the nodes of the edges just ahead
+//   k_uw_merge_lds     images of at most 65536 pixels: one wavefront per
+//                      image merges groups edge by edge in sorted order
+//                      (union by size with whole-turn offsets, the
+//                      published merge rules), group sizes in LDS, the
+//                      root finds of 64 edges at a time across the lanes
+//   k_uw_gather        larger images: one thread per image, the groups in
+//                      global memory; a helper wavefront warms L2 with the
+//                      nodes of the edges just ahead
 //   k_uw_apply         value + 2 pi * increment
 //   k_min_*, k_align   align_unwrapped: steps of the unwrapped corner
 //                      samples along the angles, their mode per angle,
 //                      whole turns from the minimum (divmod_neg)
 //   k_finish           u0 * (log|.| + i (phase - remove)) or phase - remove
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 
 #include <rocprim/device/device_segmented_radix_sort.hpp>
 
@@ -292,6 +298,173 @@ __global__ void __launch_bounds__(kUwThreads)
                      __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Images of at most 65536 pixels: the same merges in the same order, with
+// the group sizes in LDS.  Three facts of the merge rules above carry it:
+// a root's own increment stays 0 (every node starts at 0 and a kept root
+// keeps its value), so a merge needs only each endpoint's increment
+// relative to its root; the increment of a pixel relative to any of its
+// ancestors never changes afterwards (a merge shifts a whole group), so a
+// link read at any time stays valid; and union by size keeps every pixel
+// within 16 links of its root.
+//   gsz[p] (LDS, 16 bits): p's group size less one while p is a root,
+//   kUwNotRoot once it is not (saturated at 0xFFFE, which only the whole
+//   image of 65536 pixels reaches, at its last merge).
+// Per batch of 64 sorted edges, lane k walks both ends of edge k to their
+// current roots through the global links (pointing each node it passes at
+// its grandparent: only non-root nodes are written, never lane 0's), from
+// the first links loaded before the previous batch's merges.  Then the
+// wave applies the batch's merges in order with lane 0 writing: equal
+// roots are skipped without an LDS access; a root that an earlier merge of
+// the batch demoted reads kUwNotRoot and is walked on from; a merge writes
+// the demoted root's link (global) and the two size words (LDS).
+constexpr int kUwLdsPix = 65536;
+constexpr unsigned short kUwNotRoot = 0xFFFF;
+
+extern __shared__ __align__(16) unsigned char uw_smem[];
+
+// links as single 64-bit accesses (x = parent or -1 at a root, y = offset)
+__device__ __forceinline__ int2 uw_ld(int2 *node, int p) {
+  const unsigned long long v =
+      __hip_atomic_load(reinterpret_cast<unsigned long long *>(node) + p,
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
+}
+__device__ __forceinline__ void uw_st(int2 *node, int p, int parent, int off) {
+  const unsigned long long v =
+      (unsigned long long)(unsigned)parent | ((unsigned long long)(unsigned)off << 32);
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(node) + p, v,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Roots of p1 and p2 and their increments relative to them, the two walks
+// in step (their loads in flight together); pre1/pre2: the links as loaded
+// earlier (x < 0 if the pixel was a root then).
+__device__ __forceinline__ void uw_walk2(int p1, int2 pre1, int p2, int2 pre2,
+                                         int2 *node, const unsigned short *gsz,
+                                         int *r1o, int *s1o, int *r2o, int *s2o) {
+  bool d1 = gsz[p1] != kUwNotRoot, d2 = gsz[p2] != kUwNotRoot;
+  int r1 = p1, r2 = p2, s1 = 0, s2 = 0;
+  int2 v1 = pre1, v2 = pre2;
+  if (!d1 && pre1.x < 0) v1 = uw_ld(node, p1);
+  if (!d2 && pre2.x < 0) v2 = uw_ld(node, p2);
+  for (int d = 0; d < 64 && !(d1 && d2); ++d) {  // at most 16 links each
+    // v1/v2: the links of the non-root nodes r1/r2
+    bool up1 = false, up2 = false;
+    int q1 = 0, q2 = 0;
+    if (!d1) {
+      q1 = v1.x;
+      s1 += v1.y;
+      up1 = gsz[q1] == kUwNotRoot;
+    }
+    if (!d2) {
+      q2 = v2.x;
+      s2 += v2.y;
+      up2 = gsz[q2] == kUwNotRoot;
+    }
+    int2 w1 = v1, w2 = v2;
+    if (up1) w1 = uw_ld(node, q1);
+    if (up2) w2 = uw_ld(node, q2);
+    if (!d1) {
+      if (up1) uw_st(node, r1, w1.x, v1.y + w1.y);  // r1 skips q1
+      d1 = !up1;
+      r1 = q1;
+      v1 = w1;
+    }
+    if (!d2) {
+      if (up2) uw_st(node, r2, w2.x, v2.y + w2.y);
+      d2 = !up2;
+      r2 = q2;
+      v2 = w2;
+    }
+  }
+  *r1o = r1;
+  *s1o = s1;
+  *r2o = r2;
+  *s2o = s2;
+}
+
+__global__ void __launch_bounds__(64)
+    k_uw_merge_lds(int A, int H, int W, const int *__restrict__ sorted,
+                   int2 *node_all) {
+  const int a = blockIdx.x;
+  if (a >= A) return;  // uniform per block
+  const int npix = H * W;
+  const int E = H * (W - 1) + (H - 1) * W;
+  int2 *node = node_all + (long long)a * npix;
+  unsigned short *gsz = reinterpret_cast<unsigned short *>(uw_smem);
+  const int lane = threadIdx.x;
+  for (int p = lane; p < npix; p += 64) {
+    uw_st(node, p, -1, 0);
+    gsz[p] = 0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+  const int *es = sorted + (long long)a * E;
+  auto ends = [&](int ev, int &q1, int &q2) {
+    q1 = q2 = 0;
+    if (ev >= 0) uw_edge(ev >> 2, H, W, &q1, &q2);
+  };
+  int ev = lane < E ? es[lane] : -1;
+  int ev1 = 64 + lane < E ? es[64 + lane] : -1;
+  int p1, p2;
+  ends(ev, p1, p2);
+  int2 pre1 = make_int2(-1, 0), pre2 = make_int2(-1, 0);  // all roots at first
+  for (int k0 = 0; k0 < E; k0 += 64) {
+    int r1 = 0, r2 = 0, s1 = 0, s2 = 0;
+    if (ev >= 0) uw_walk2(p1, pre1, p2, pre2, node, gsz, &r1, &s1, &r2, &s2);
+    // the next batch's first links and the edges of the batch after it,
+    // in flight during this batch's merges
+    int q1, q2;
+    ends(ev1, q1, q2);
+    const int2 n1 = ev1 >= 0 ? uw_ld(node, q1) : make_int2(-1, 0);
+    const int2 n2 = ev1 >= 0 ? uw_ld(node, q2) : make_int2(-1, 0);
+    const int ev2 = k0 + 128 + lane < E ? es[k0 + 128 + lane] : -1;
+    for (int k = 0; k < 64; ++k) {
+      const int ek = __builtin_amdgcn_readlane(ev, k);
+      if (ek < 0) break;
+      int a1 = __builtin_amdgcn_readlane(r1, k), a2 = __builtin_amdgcn_readlane(r2, k);
+      if (a1 == a2) continue;
+      int t1 = __builtin_amdgcn_readlane(s1, k), t2 = __builtin_amdgcn_readlane(s2, k);
+      unsigned g1 = gsz[a1], g2 = gsz[a2];
+      if (g1 == kUwNotRoot || g2 == kUwNotRoot) {
+        __builtin_amdgcn_s_waitcnt(0);  // this batch's links are in memory
+        for (int d = 0; d < 64 && g1 == kUwNotRoot; ++d) {
+          const int2 v = uw_ld(node, a1);
+          t1 += v.y;
+          a1 = v.x;
+          g1 = gsz[a1];
+        }
+        for (int d = 0; d < 64 && g2 == kUwNotRoot; ++d) {
+          const int2 v = uw_ld(node, a2);
+          t2 += v.y;
+          a2 = v.x;
+          g2 = gsz[a2];
+        }
+        if (a1 == a2) continue;
+      }
+      const int e = (ek & 3) - 1;
+      // group 2 joins group 1 when it is a single pixel or the smaller one
+      const bool keep1 = g2 == 0 || g1 > g2;
+      const int child = keep1 ? a2 : a1, target = keep1 ? a1 : a2;
+      const int off = keep1 ? t1 - e - t2 : t2 + e - t1;
+      const unsigned tot = g1 + g2 + 1;
+      if (lane == 0) {
+        gsz[target] = (unsigned short)(tot < 0xFFFEu ? tot : 0xFFFEu);
+        gsz[child] = kUwNotRoot;
+        uw_st(node, child, target, off);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);  // the merges' links before the next walks
+    ev = ev1;
+    ev1 = ev2;
+    p1 = q1;
+    p2 = q2;
+    pre1 = n1;
+    pre2 = n2;
+  }
+}
+
 // value + 2 pi * increment, in float64 as the published code, stored as R
 template <typename R>
 __global__ void __launch_bounds__(256)
@@ -531,9 +704,20 @@ hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const vo
                                             0, 64, st);
     if (e != hipSuccess) return e;
     int2 *node = (int2 *)(w + L.parent);
-    // one image per wavefront: images advance independently (no lanes
-    // waiting on each other's longer find paths)
-    k_uw_gather<R><<<A, kUwThreads, 0, st>>>(A, H, W, v1, node);
+    // one image per block: images advance independently.  Images of at
+    // most 65536 pixels keep their group sizes in LDS (ODTBRAIN_AMD_UW_GLOBAL
+    // set: the global union-find for every size)
+    if ((long long)H * W <= kUwLdsPix && !std::getenv("ODTBRAIN_AMD_UW_GLOBAL")) {
+      const size_t sh = (size_t)H * W * sizeof(unsigned short);
+      if (sh > 65536 &&
+          (e = hipFuncSetAttribute((const void *)k_uw_merge_lds,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh)) !=
+              hipSuccess)
+        return e;
+      k_uw_merge_lds<<<A, 64, sh, st>>>(A, H, W, v1, node);
+    } else {
+      k_uw_gather<R><<<A, kUwThreads, 0, st>>>(A, H, W, v1, node);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_uw_apply<R><<<blocks_of(n), 256, 0, st>>>(phase, A, H, W, node);
     if ((e = hipGetLastError()) != hipSuccess) return e;

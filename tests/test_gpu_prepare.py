@@ -89,6 +89,24 @@ def test_prepare_matches_oracle_with_residues(shape, align):
     assert np.allclose(got_r, ref_r, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("shape", [(4, 256, 256), (3, 97, 130)])
+@pytest.mark.parametrize("align", [True, False])
+def test_unwrap_lds_merge_equals_global_merge(shape, align, monkeypatch):
+    """Images of at most 65536 pixels merge with their group sizes in LDS,
+    64 root finds at a time; the global union-find (one thread per image)
+    applies the same merges in the same order: the same output bits, on
+    noisy fields full of residues."""
+    u = _noisy_field(shape, seed=11, noise=1.2)
+    assert _residues(np.angle(u[0])) > 500
+    got = G.sinogram_as_rytov(u, align=align)
+    monkeypatch.setenv("ODTBRAIN_AMD_UW_GLOBAL", "1")
+    ref = G.sinogram_as_rytov(u, align=align)
+    assert np.array_equal(got.view(float), ref.view(float))
+    got_r = G.sinogram_as_radon(u[:2], align=align)
+    monkeypatch.delenv("ODTBRAIN_AMD_UW_GLOBAL")
+    assert np.array_equal(got_r, G.sinogram_as_radon(u[:2], align=align))
+
+
 def test_prepare_u0_array_and_float32_and_tensors():
     import torch
     u = _noisy_field((3, 10, 12), seed=3, noise=0.2)
