@@ -313,8 +313,8 @@ __global__ void __launch_bounds__(kUwThreads)
 // current roots through the global links (pointing each node it passes at
 // its grandparent: only non-root nodes are written, never lane 0's), from
 // the first links loaded before the previous batch's merges.  Then the
-// wave applies the batch's merges in order with lane 0 writing: equal
-// roots are skipped without an LDS access; a root that an earlier merge of
+// wave applies the batch's merges in order with lane 0 writing: it visits
+// only the edges whose ends had different roots at the walk; a root that an earlier merge of
 // the batch demoted reads kUwNotRoot and is walked on from; a merge writes
 // the demoted root's link (global) and the two size words (LDS).
 constexpr int kUwLdsPix = 65536;
@@ -419,11 +419,14 @@ __global__ void __launch_bounds__(64)
     const int2 n1 = ev1 >= 0 ? uw_ld(node, q1) : make_int2(-1, 0);
     const int2 n2 = ev1 >= 0 ? uw_ld(node, q2) : make_int2(-1, 0);
     const int ev2 = k0 + 128 + lane < E ? es[k0 + 128 + lane] : -1;
-    for (int k = 0; k < 64; ++k) {
+    // edges whose ends had different roots at their walk (an edge with one
+    // root then has it for good); the others merge nothing
+    unsigned long long cand = __ballot(ev >= 0 && r1 != r2);
+    while (cand) {
+      const int k = __builtin_ctzll(cand);
+      cand &= cand - 1;
       const int ek = __builtin_amdgcn_readlane(ev, k);
-      if (ek < 0) break;
       int a1 = __builtin_amdgcn_readlane(r1, k), a2 = __builtin_amdgcn_readlane(r2, k);
-      if (a1 == a2) continue;
       int t1 = __builtin_amdgcn_readlane(s1, k), t2 = __builtin_amdgcn_readlane(s2, k);
       unsigned g1 = gsz[a1], g2 = gsz[a2];
       if (g1 == kUwNotRoot || g2 == kUwNotRoot) {
