@@ -307,20 +307,25 @@ __global__ void __launch_bounds__(kUwThreads)
 // link read at any time stays valid; and union by size keeps every pixel
 // within 16 links of its root.
 //   gsz[p] (LDS, 16 bits): p's group size less one while p is a root,
-//   kUwNotRoot once it is not (saturated at 0xFFFE, which only the whole
-//   image of 65536 pixels reaches, at its last merge).
+//   kUwNotRoot once it is not; kUwSat once the group holds more than half
+//   the pixels (it then wins every size comparison, as its true size would).
 // Per batch of 64 sorted edges, lane k walks both ends of edge k to their
 // current roots through the global links (pointing each node it passes at
-// its grandparent: only non-root nodes are written, never lane 0's), from
-// the first links loaded before the previous batch's merges.  Then the
-// wave applies the batch's merges in order with lane 0 writing: it visits
-// only the edges whose ends had different roots at the walk; a root that an earlier merge of
-// the batch demoted reads kUwNotRoot and is walked on from; a merge writes
-// the demoted root's link (global) and the two size words (LDS).
+// its grandparent: only non-root nodes are written, never by a merge),
+// from the first links loaded before the previous batch's merges; then the
+// batch's merges run in rounds of conflict-free edges (below), the rest in
+// edge order.  A merge writes the demoted root's link (global) and the two
+// size words (LDS).
 constexpr int kUwLdsPix = 65536;
+constexpr unsigned short kUwSat = 0xFFFE;  // majority group (more than half the pixels)
+constexpr int kUwClaims = 2048;            // claim slots (LDS, after the size words)
+constexpr int kUwMinWinners = 2;
 constexpr unsigned short kUwNotRoot = 0xFFFF;
 
 extern __shared__ __align__(16) unsigned char uw_smem[];
+__host__ __device__ constexpr size_t uw_claims_offset(long long npix) {
+  return ((size_t)npix * sizeof(unsigned short) + 15) / 16 * 16;
+}
 
 // links as single 64-bit accesses (x = parent or -1 at a root, y = offset)
 __device__ __forceinline__ int2 uw_ld(int2 *node, int p) {
@@ -392,6 +397,7 @@ __global__ void __launch_bounds__(64)
   const int E = H * (W - 1) + (H - 1) * W;
   int2 *node = node_all + (long long)a * npix;
   unsigned short *gsz = reinterpret_cast<unsigned short *>(uw_smem);
+  unsigned *claim = reinterpret_cast<unsigned *>(uw_smem + uw_claims_offset(npix));
   const int lane = threadIdx.x;
   for (int p = lane; p < npix; p += 64) {
     uw_st(node, p, -1, 0);
@@ -419,18 +425,26 @@ __global__ void __launch_bounds__(64)
     const int2 n1 = ev1 >= 0 ? uw_ld(node, q1) : make_int2(-1, 0);
     const int2 n2 = ev1 >= 0 ? uw_ld(node, q2) : make_int2(-1, 0);
     const int ev2 = k0 + 128 + lane < E ? es[k0 + 128 + lane] : -1;
-    // edges whose ends had different roots at their walk (an edge with one
-    // root then has it for good); the others merge nothing
-    unsigned long long cand = __ballot(ev >= 0 && r1 != r2);
-    while (cand) {
-      const int k = __builtin_ctzll(cand);
-      cand &= cand - 1;
-      const int ek = __builtin_amdgcn_readlane(ev, k);
-      int a1 = __builtin_amdgcn_readlane(r1, k), a2 = __builtin_amdgcn_readlane(r2, k);
-      int t1 = __builtin_amdgcn_readlane(s1, k), t2 = __builtin_amdgcn_readlane(s2, k);
-      unsigned g1 = gsz[a1], g2 = gsz[a2];
-      if (g1 == kUwNotRoot || g2 == kUwNotRoot) {
-        __builtin_amdgcn_s_waitcnt(0);  // this batch's links are in memory
+    // The batch's merges, in rounds: an edge whose ends have different
+    // current roots claims each root below the majority size (atomic min of
+    // its lane in a hashed table); the edges holding all their claims merge
+    // at once.  An earlier edge sharing a root with a later one holds that
+    // root's claim, so no merge changes the roots or sizes an earlier
+    // pending edge will see; merges into the majority group (more than half
+    // the pixels: it wins every size comparison, its size word saturated)
+    // commute.  The lowest pending edge always merges; a round with fewer
+    // than kUwMinWinners merges hands the rest to the in-order loop below.
+    int a1 = r1, a2 = r2, t1 = s1, t2 = s2;
+    const int e = (ev & 3) - 1;
+    const unsigned thr = (unsigned)npix / 2;
+    unsigned long long pend = __ballot(ev >= 0 && r1 != r2);
+    for (int round = 0; pend != 0 && round < 64; ++round) {
+      const bool me = (pend >> lane) & 1ull;
+      unsigned g1 = 0, g2 = 0;
+      if (me) {
+        // roots demoted by an earlier round link to current ones
+        g1 = gsz[a1];
+        g2 = gsz[a2];
         for (int d = 0; d < 64 && g1 == kUwNotRoot; ++d) {
           const int2 v = uw_ld(node, a1);
           t1 += v.y;
@@ -443,18 +457,66 @@ __global__ void __launch_bounds__(64)
           a2 = v.x;
           g2 = gsz[a2];
         }
-        if (a1 == a2) continue;
       }
-      const int e = (ek & 3) - 1;
+      const bool live = me && a1 != a2;
+      pend = __ballot(live);
+      if (pend == 0) break;
+      const int h1 = a1 & (kUwClaims - 1), h2 = a2 & (kUwClaims - 1);
+      const bool c1 = live && g1 != kUwSat, c2 = live && g2 != kUwSat;
+      if (c1) claim[h1] = ~0u;
+      if (c2) claim[h2] = ~0u;
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the LDS runs a wave's operations in order
+      if (c1) atomicMin(&claim[h1], (unsigned)lane);
+      if (c2) atomicMin(&claim[h2], (unsigned)lane);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const bool win = live && (!c1 || claim[h1] == (unsigned)lane) &&
+                       (!c2 || claim[h2] == (unsigned)lane);
+      const unsigned long long wins = __ballot(win);
+      if (__builtin_popcountll(wins) < kUwMinWinners) break;
+      if (win) {
+        const bool keep1 = g2 == 0 || g1 > g2;
+        const int child = keep1 ? a2 : a1, target = keep1 ? a1 : a2;
+        const unsigned tot = g1 + g2 + 1;
+        gsz[target] = (unsigned short)(tot >= thr ? kUwSat : tot);
+        gsz[child] = kUwNotRoot;
+        uw_st(node, child, target, keep1 ? t1 - e - t2 : t2 + e - t1);
+      }
+      pend &= ~wins;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_s_waitcnt(0);  // the round's links before the next walks
+    }
+    // the rest in edge order, lane 0 writing
+    while (pend) {
+      const int k = __builtin_ctzll(pend);
+      pend &= pend - 1;
+      int b1 = __builtin_amdgcn_readlane(a1, k), b2 = __builtin_amdgcn_readlane(a2, k);
+      int u1 = __builtin_amdgcn_readlane(t1, k), u2 = __builtin_amdgcn_readlane(t2, k);
+      const int ek = __builtin_amdgcn_readlane(e, k);
+      unsigned g1 = gsz[b1], g2 = gsz[b2];
+      if (g1 == kUwNotRoot || g2 == kUwNotRoot) {
+        __builtin_amdgcn_s_waitcnt(0);  // this loop's links are in memory
+        for (int d = 0; d < 64 && g1 == kUwNotRoot; ++d) {
+          const int2 v = uw_ld(node, b1);
+          u1 += v.y;
+          b1 = v.x;
+          g1 = gsz[b1];
+        }
+        for (int d = 0; d < 64 && g2 == kUwNotRoot; ++d) {
+          const int2 v = uw_ld(node, b2);
+          u2 += v.y;
+          b2 = v.x;
+          g2 = gsz[b2];
+        }
+      }
+      if (b1 == b2) continue;
       // group 2 joins group 1 when it is a single pixel or the smaller one
       const bool keep1 = g2 == 0 || g1 > g2;
-      const int child = keep1 ? a2 : a1, target = keep1 ? a1 : a2;
-      const int off = keep1 ? t1 - e - t2 : t2 + e - t1;
+      const int child = keep1 ? b2 : b1, target = keep1 ? b1 : b2;
       const unsigned tot = g1 + g2 + 1;
       if (lane == 0) {
-        gsz[target] = (unsigned short)(tot < 0xFFFEu ? tot : 0xFFFEu);
+        gsz[target] = (unsigned short)(tot >= thr ? kUwSat : tot);
         gsz[child] = kUwNotRoot;
-        uw_st(node, child, target, off);
+        uw_st(node, child, target, keep1 ? u1 - ek - u2 : u2 + ek - u1);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -711,7 +773,7 @@ hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const vo
     // most 65536 pixels keep their group sizes in LDS (ODTBRAIN_AMD_UW_GLOBAL
     // set: the global union-find for every size)
     if ((long long)H * W <= kUwLdsPix && !std::getenv("ODTBRAIN_AMD_UW_GLOBAL")) {
-      const size_t sh = (size_t)H * W * sizeof(unsigned short);
+      const size_t sh = uw_claims_offset((long long)H * W) + kUwClaims * sizeof(unsigned);
       if (sh > 65536 &&
           (e = hipFuncSetAttribute((const void *)k_uw_merge_lds,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh)) !=
