@@ -89,7 +89,8 @@ def test_prepare_matches_oracle_with_residues(shape, align):
     assert np.allclose(got_r, ref_r, rtol=0, atol=1e-12)
 
 
-@pytest.mark.parametrize("shape", [(4, 256, 256), (3, 97, 130)])
+@pytest.mark.parametrize("shape", [(4, 256, 256), (3, 97, 130), (1, 255, 257),
+                                   (2, 2, 50), (2, 50, 3), (2, 2, 3)])
 @pytest.mark.parametrize("align", [True, False])
 def test_unwrap_lds_merge_equals_global_merge(shape, align, monkeypatch):
     """Images of at most 65536 pixels merge with their group sizes in LDS,
@@ -97,7 +98,8 @@ def test_unwrap_lds_merge_equals_global_merge(shape, align, monkeypatch):
     applies the same merges in the same order: the same output bits, on
     noisy fields full of residues."""
     u = _noisy_field(shape, seed=11, noise=1.2)
-    assert _residues(np.angle(u[0])) > 500
+    if shape[1] * shape[2] >= 10000:
+        assert _residues(np.angle(u[0])) > 500
     got = G.sinogram_as_rytov(u, align=align)
     monkeypatch.setenv("ODTBRAIN_AMD_UW_GLOBAL", "1")
     ref = G.sinogram_as_rytov(u, align=align)
