@@ -309,13 +309,13 @@ __global__ void __launch_bounds__(kUwThreads)
 //   gsz[p] (LDS, 16 bits): p's group size less one while p is a root,
 //   kUwNotRoot once it is not; kUwSat once the group holds more than half
 //   the pixels (it then wins every size comparison, as its true size would).
-// Per batch of 64 sorted edges, lane k walks both ends of edge k to their
-// current roots through the global links (pointing each node it passes at
-// its grandparent: only non-root nodes are written, never by a merge),
-// from the first links loaded before the previous batch's merges; then the
-// batch's merges run in rounds of conflict-free edges (below), the rest in
-// edge order.  A merge writes the demoted root's link (global) and the two
-// size words (LDS).
+// Per batch of 64 sorted edges, lane k of one wave walks both ends of edge k
+// to their roots through the global links (pointing each node it passes at
+// its grandparent: only nodes with a link are written, never by a merge),
+// from the first links loaded a batch earlier; the other wave then merges
+// the batch, in rounds of conflict-free edges and the rest in edge order,
+// while the next batch is walked.  A merge writes the demoted root's link
+// (global) and the two size words (LDS).
 constexpr int kUwLdsPix = 65536;
 constexpr unsigned short kUwSat = 0xFFFE;  // majority group (more than half the pixels)
 constexpr int kUwClaims = 2048;            // claim slots (LDS, after the size words)
@@ -352,6 +352,10 @@ __device__ __forceinline__ void uw_walk2(int p1, int2 pre1, int p2, int2 pre2,
   int2 v1 = pre1, v2 = pre2;
   if (!d1 && pre1.x < 0) v1 = uw_ld(node, p1);
   if (!d2 && pre2.x < 0) v2 = uw_ld(node, p2);
+  // a link that still reads as a root (its merge's store not yet in
+  // memory): the walk stops there and the merging side walks on
+  d1 = d1 || v1.x < 0;
+  d2 = d2 || v2.x < 0;
   for (int d = 0; d < 64 && !(d1 && d2); ++d) {  // at most 16 links each
     // v1/v2: the links of the non-root nodes r1/r2
     bool up1 = false, up2 = false;
@@ -370,14 +374,16 @@ __device__ __forceinline__ void uw_walk2(int p1, int2 pre1, int p2, int2 pre2,
     if (up1) w1 = uw_ld(node, q1);
     if (up2) w2 = uw_ld(node, q2);
     if (!d1) {
-      if (up1) uw_st(node, r1, w1.x, v1.y + w1.y);  // r1 skips q1
-      d1 = !up1;
+      const bool go = up1 && w1.x >= 0;
+      if (go) uw_st(node, r1, w1.x, v1.y + w1.y);  // r1 skips q1
+      d1 = !go;
       r1 = q1;
       v1 = w1;
     }
     if (!d2) {
-      if (up2) uw_st(node, r2, w2.x, v2.y + w2.y);
-      d2 = !up2;
+      const bool go = up2 && w2.x >= 0;
+      if (go) uw_st(node, r2, w2.x, v2.y + w2.y);
+      d2 = !go;
       r2 = q2;
       v2 = w2;
     }
@@ -388,7 +394,27 @@ __device__ __forceinline__ void uw_walk2(int p1, int2 pre1, int p2, int2 pre2,
   *s2o = s2;
 }
 
-__global__ void __launch_bounds__(64)
+// On from a root that a merge demoted (its link is in memory: the merging
+// wave's stores complete before each round and batch ends); false if a
+// link still reads as a root.
+__device__ __forceinline__ void uw_resolve(int2 *node, const unsigned short *gsz, int &a,
+                                           int &t, unsigned &g) {
+  for (int d = 0; d < 64 && g == kUwNotRoot; ++d) {
+    const int2 v = uw_ld(node, a);
+    if (v.x < 0) break;
+    t += v.y;
+    a = v.x;
+    g = gsz[a];
+  }
+}
+
+// Two wavefronts per image: wave 1 walks the ends of batch i to their roots
+// (and loads the first links of batch i + 1) while wave 0 merges batch
+// i - 1; the walk results pass through an LDS ring of two slots and both
+// waves meet at one barrier per batch.  The walks see the merges of the
+// batch before only in part: a root they return may since have been demoted
+// (its size word reads kUwNotRoot, and the merging wave walks on from it).
+__global__ void __launch_bounds__(128)
     k_uw_merge_lds(int A, int H, int W, const int *__restrict__ sorted,
                    int2 *node_all) {
   const int a = blockIdx.x;
@@ -398,135 +424,130 @@ __global__ void __launch_bounds__(64)
   int2 *node = node_all + (long long)a * npix;
   unsigned short *gsz = reinterpret_cast<unsigned short *>(uw_smem);
   unsigned *claim = reinterpret_cast<unsigned *>(uw_smem + uw_claims_offset(npix));
-  const int lane = threadIdx.x;
-  for (int p = lane; p < npix; p += 64) {
+  int *ring = reinterpret_cast<int *>(claim + kUwClaims);  // [2][5][64]
+  const int lane = threadIdx.x & 63;
+  const bool walker = threadIdx.x >= 64;  // wave 1
+  for (int p = threadIdx.x; p < npix; p += 128) {
     uw_st(node, p, -1, 0);
     gsz[p] = 0;
   }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
   const int *es = sorted + (long long)a * E;
   auto ends = [&](int ev, int &q1, int &q2) {
     q1 = q2 = 0;
     if (ev >= 0) uw_edge(ev >> 2, H, W, &q1, &q2);
   };
-  int ev = lane < E ? es[lane] : -1;
-  int ev1 = 64 + lane < E ? es[64 + lane] : -1;
-  int p1, p2;
-  ends(ev, p1, p2);
-  int2 pre1 = make_int2(-1, 0), pre2 = make_int2(-1, 0);  // all roots at first
-  for (int k0 = 0; k0 < E; k0 += 64) {
-    int r1 = 0, r2 = 0, s1 = 0, s2 = 0;
-    if (ev >= 0) uw_walk2(p1, pre1, p2, pre2, node, gsz, &r1, &s1, &r2, &s2);
-    // the next batch's first links and the edges of the batch after it,
-    // in flight during this batch's merges
-    int q1, q2;
-    ends(ev1, q1, q2);
-    const int2 n1 = ev1 >= 0 ? uw_ld(node, q1) : make_int2(-1, 0);
-    const int2 n2 = ev1 >= 0 ? uw_ld(node, q2) : make_int2(-1, 0);
-    const int ev2 = k0 + 128 + lane < E ? es[k0 + 128 + lane] : -1;
-    // The batch's merges, in rounds: an edge whose ends have different
-    // current roots claims each root below the majority size (atomic min of
-    // its lane in a hashed table); the edges holding all their claims merge
-    // at once.  An earlier edge sharing a root with a later one holds that
-    // root's claim, so no merge changes the roots or sizes an earlier
-    // pending edge will see; merges into the majority group (more than half
-    // the pixels: it wins every size comparison, its size word saturated)
-    // commute.  The lowest pending edge always merges; a round with fewer
-    // than kUwMinWinners merges hands the rest to the in-order loop below.
-    int a1 = r1, a2 = r2, t1 = s1, t2 = s2;
-    const int e = (ev & 3) - 1;
-    const unsigned thr = (unsigned)npix / 2;
-    unsigned long long pend = __ballot(ev >= 0 && r1 != r2);
-    for (int round = 0; pend != 0 && round < 64; ++round) {
-      const bool me = (pend >> lane) & 1ull;
-      unsigned g1 = 0, g2 = 0;
-      if (me) {
-        // roots demoted by an earlier round link to current ones
-        g1 = gsz[a1];
-        g2 = gsz[a2];
-        for (int d = 0; d < 64 && g1 == kUwNotRoot; ++d) {
-          const int2 v = uw_ld(node, a1);
-          t1 += v.y;
-          a1 = v.x;
+  const int nb = (E + 63) / 64;
+  const unsigned thr = (unsigned)npix / 2;
+  // the walking wave's state: this batch's and the next one's edges, the
+  // ends and their first links (all roots at first)
+  int wev = -1, wev1 = -1, p1 = 0, p2 = 0;
+  int2 pre1 = make_int2(-1, 0), pre2 = make_int2(-1, 0);
+  if (walker) {
+    wev = lane < E ? es[lane] : -1;
+    wev1 = 64 + lane < E ? es[64 + lane] : -1;
+    ends(wev, p1, p2);
+  }
+  for (int i = 0; i <= nb; ++i) {  // both waves: nb + 1 barriers
+    if (walker) {
+      if (i < nb) {
+        int r1 = 0, r2 = 0, s1 = 0, s2 = 0;
+        if (wev >= 0) uw_walk2(p1, pre1, p2, pre2, node, gsz, &r1, &s1, &r2, &s2);
+        int *slot = ring + (i & 1) * 320;
+        slot[lane] = wev;
+        slot[64 + lane] = r1;
+        slot[128 + lane] = s1;
+        slot[192 + lane] = r2;
+        slot[256 + lane] = s2;
+        int q1, q2;
+        ends(wev1, q1, q2);
+        pre1 = wev1 >= 0 ? uw_ld(node, q1) : make_int2(-1, 0);
+        pre2 = wev1 >= 0 ? uw_ld(node, q2) : make_int2(-1, 0);
+        const int k2 = (i + 2) * 64 + lane;
+        wev = wev1;
+        wev1 = k2 < E ? es[k2] : -1;
+        p1 = q1;
+        p2 = q2;
+      }
+    } else if (i >= 1) {
+      const int *slot = ring + ((i - 1) & 1) * 320;
+      const int ev = slot[lane];
+      int a1 = slot[64 + lane], t1 = slot[128 + lane];
+      int a2 = slot[192 + lane], t2 = slot[256 + lane];
+      const int e = (ev & 3) - 1;
+      // The batch's merges, in rounds: an edge whose ends have different
+      // current roots claims each root below the majority size (atomic min
+      // of its lane in a hashed table); the edges holding all their claims
+      // merge at once.  An earlier edge sharing a root with a later one
+      // holds that root's claim, so no merge changes the roots or sizes an
+      // earlier pending edge will see; merges into the majority group (more
+      // than half the pixels: it wins every size comparison, its size word
+      // saturated) commute.  The lowest pending edge always merges; a round
+      // with fewer than kUwMinWinners merges hands the rest to the in-order
+      // loop below.
+      unsigned long long pend = __ballot(ev >= 0 && a1 != a2);
+      for (int round = 0; pend != 0 && round < 64; ++round) {
+        const bool me = (pend >> lane) & 1ull;
+        unsigned g1 = 0, g2 = 0;
+        if (me) {
           g1 = gsz[a1];
-        }
-        for (int d = 0; d < 64 && g2 == kUwNotRoot; ++d) {
-          const int2 v = uw_ld(node, a2);
-          t2 += v.y;
-          a2 = v.x;
           g2 = gsz[a2];
+          uw_resolve(node, gsz, a1, t1, g1);
+          uw_resolve(node, gsz, a2, t2, g2);
         }
+        const bool live = me && a1 != a2;
+        pend = __ballot(live);
+        if (pend == 0) break;
+        const int h1 = a1 & (kUwClaims - 1), h2 = a2 & (kUwClaims - 1);
+        const bool c1 = live && g1 != kUwSat, c2 = live && g2 != kUwSat;
+        if (c1) claim[h1] = ~0u;
+        if (c2) claim[h2] = ~0u;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the LDS runs a wave's operations in order
+        if (c1) atomicMin(&claim[h1], (unsigned)lane);
+        if (c2) atomicMin(&claim[h2], (unsigned)lane);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const bool win = live && (!c1 || claim[h1] == (unsigned)lane) &&
+                         (!c2 || claim[h2] == (unsigned)lane);
+        const unsigned long long wins = __ballot(win);
+        if (__builtin_popcountll(wins) < kUwMinWinners) break;
+        if (win) {
+          const bool keep1 = g2 == 0 || g1 > g2;
+          const int child = keep1 ? a2 : a1, target = keep1 ? a1 : a2;
+          const unsigned tot = g1 + g2 + 1;
+          gsz[target] = (unsigned short)(tot >= thr ? kUwSat : tot);
+          gsz[child] = kUwNotRoot;
+          uw_st(node, child, target, keep1 ? t1 - e - t2 : t2 + e - t1);
+        }
+        pend &= ~wins;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);  // the round's links before the next walks
       }
-      const bool live = me && a1 != a2;
-      pend = __ballot(live);
-      if (pend == 0) break;
-      const int h1 = a1 & (kUwClaims - 1), h2 = a2 & (kUwClaims - 1);
-      const bool c1 = live && g1 != kUwSat, c2 = live && g2 != kUwSat;
-      if (c1) claim[h1] = ~0u;
-      if (c2) claim[h2] = ~0u;
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the LDS runs a wave's operations in order
-      if (c1) atomicMin(&claim[h1], (unsigned)lane);
-      if (c2) atomicMin(&claim[h2], (unsigned)lane);
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      const bool win = live && (!c1 || claim[h1] == (unsigned)lane) &&
-                       (!c2 || claim[h2] == (unsigned)lane);
-      const unsigned long long wins = __ballot(win);
-      if (__builtin_popcountll(wins) < kUwMinWinners) break;
-      if (win) {
+      // the rest in edge order, lane 0 writing
+      while (pend) {
+        const int k = __builtin_ctzll(pend);
+        pend &= pend - 1;
+        int b1 = __builtin_amdgcn_readlane(a1, k), b2 = __builtin_amdgcn_readlane(a2, k);
+        int u1 = __builtin_amdgcn_readlane(t1, k), u2 = __builtin_amdgcn_readlane(t2, k);
+        const int ek = __builtin_amdgcn_readlane(e, k);
+        unsigned g1 = gsz[b1], g2 = gsz[b2];
+        if (g1 == kUwNotRoot || g2 == kUwNotRoot) {
+          __builtin_amdgcn_s_waitcnt(0);  // this loop's links are in memory
+          uw_resolve(node, gsz, b1, u1, g1);
+          uw_resolve(node, gsz, b2, u2, g2);
+        }
+        if (b1 == b2) continue;
+        // group 2 joins group 1 when it is a single pixel or the smaller one
         const bool keep1 = g2 == 0 || g1 > g2;
-        const int child = keep1 ? a2 : a1, target = keep1 ? a1 : a2;
+        const int child = keep1 ? b2 : b1, target = keep1 ? b1 : b2;
         const unsigned tot = g1 + g2 + 1;
-        gsz[target] = (unsigned short)(tot >= thr ? kUwSat : tot);
-        gsz[child] = kUwNotRoot;
-        uw_st(node, child, target, keep1 ? t1 - e - t2 : t2 + e - t1);
-      }
-      pend &= ~wins;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      __builtin_amdgcn_s_waitcnt(0);  // the round's links before the next walks
-    }
-    // the rest in edge order, lane 0 writing
-    while (pend) {
-      const int k = __builtin_ctzll(pend);
-      pend &= pend - 1;
-      int b1 = __builtin_amdgcn_readlane(a1, k), b2 = __builtin_amdgcn_readlane(a2, k);
-      int u1 = __builtin_amdgcn_readlane(t1, k), u2 = __builtin_amdgcn_readlane(t2, k);
-      const int ek = __builtin_amdgcn_readlane(e, k);
-      unsigned g1 = gsz[b1], g2 = gsz[b2];
-      if (g1 == kUwNotRoot || g2 == kUwNotRoot) {
-        __builtin_amdgcn_s_waitcnt(0);  // this loop's links are in memory
-        for (int d = 0; d < 64 && g1 == kUwNotRoot; ++d) {
-          const int2 v = uw_ld(node, b1);
-          u1 += v.y;
-          b1 = v.x;
-          g1 = gsz[b1];
-        }
-        for (int d = 0; d < 64 && g2 == kUwNotRoot; ++d) {
-          const int2 v = uw_ld(node, b2);
-          u2 += v.y;
-          b2 = v.x;
-          g2 = gsz[b2];
+        if (lane == 0) {
+          gsz[target] = (unsigned short)(tot >= thr ? kUwSat : tot);
+          gsz[child] = kUwNotRoot;
+          uw_st(node, child, target, keep1 ? u1 - ek - u2 : u2 + ek - u1);
         }
       }
-      if (b1 == b2) continue;
-      // group 2 joins group 1 when it is a single pixel or the smaller one
-      const bool keep1 = g2 == 0 || g1 > g2;
-      const int child = keep1 ? b2 : b1, target = keep1 ? b1 : b2;
-      const unsigned tot = g1 + g2 + 1;
-      if (lane == 0) {
-        gsz[target] = (unsigned short)(tot >= thr ? kUwSat : tot);
-        gsz[child] = kUwNotRoot;
-        uw_st(node, child, target, keep1 ? u1 - ek - u2 : u2 + ek - u1);
-      }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);  // the merges' links before the next walks
-    ev = ev1;
-    ev1 = ev2;
-    p1 = q1;
-    p2 = q2;
-    pre1 = n1;
-    pre2 = n2;
+    __syncthreads();  // merges and walk results of this step before the next
   }
 }
 
@@ -773,13 +794,14 @@ hipError_t launch_prepare(int ndim, int A, int H, int W, const void *u, const vo
     // most 65536 pixels keep their group sizes in LDS (ODTBRAIN_AMD_UW_GLOBAL
     // set: the global union-find for every size)
     if ((long long)H * W <= kUwLdsPix && !std::getenv("ODTBRAIN_AMD_UW_GLOBAL")) {
-      const size_t sh = uw_claims_offset((long long)H * W) + kUwClaims * sizeof(unsigned);
+      const size_t sh = uw_claims_offset((long long)H * W) +
+                        (kUwClaims + 2 * 5 * 64) * sizeof(unsigned);
       if (sh > 65536 &&
           (e = hipFuncSetAttribute((const void *)k_uw_merge_lds,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh)) !=
               hipSuccess)
         return e;
-      k_uw_merge_lds<<<A, 64, sh, st>>>(A, H, W, v1, node);
+      k_uw_merge_lds<<<A, 128, sh, st>>>(A, H, W, v1, node);
     } else {
       k_uw_gather<R><<<A, kUwThreads, 0, st>>>(A, H, W, v1, node);
     }
