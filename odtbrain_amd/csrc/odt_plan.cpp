@@ -7,6 +7,7 @@
 // fused B-spline prefilter terms, and the per-angle weights and geometry.
 // Execution streams the projections through the transform kernels in
 // chunks and accumulates batches of coefficient slabs into the output.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,6 +61,42 @@ bool fault_fires(int kind) {  // 0: allocation, 1: launch
   const long long n = std::atoll(g_fault_spec.c_str() + wl);
   return ++g_fault_count[kind] == n;
 }
+
+// ROCTx ranges around the stages a call enqueues (ODTBRAIN_AMD_ROCTX=1:
+// rocprofv3 --marker-trace shows them beside the kernels).  They mark the
+// host-side enqueue of each stage, the work itself runs asynchronously on
+// the stream.  The roctx library is opened on first use: no link-time
+// dependency, and nothing is called when the variable is unset.
+struct RoctxApi {
+  int (*push)(const char *) = nullptr;
+  int (*pop)() = nullptr;
+};
+const RoctxApi &roctx_api() {
+  static const RoctxApi api = [] {
+    RoctxApi a;
+    const char *env = std::getenv("ODTBRAIN_AMD_ROCTX");
+    if (!env || env[0] == '\0' || env[0] == '0') return a;
+    void *h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_GLOBAL);
+    if (!h) h = dlopen("libroctx64.so.4", RTLD_NOW | RTLD_GLOBAL);
+    if (!h) return a;
+    a.push = reinterpret_cast<int (*)(const char *)>(dlsym(h, "roctxRangePushA"));
+    a.pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+    if (!a.push || !a.pop) a = RoctxApi{};
+    return a;
+  }();
+  return api;
+}
+struct StageRange {
+  bool on;
+  explicit StageRange(const char *name) : on(roctx_api().push != nullptr) {
+    if (on) roctx_api().push(name);
+  }
+  ~StageRange() {
+    if (on) roctx_api().pop();
+  }
+  StageRange(const StageRange &) = delete;
+  StageRange &operator=(const StageRange &) = delete;
+};
 
 hipError_t dev_malloc(void **ptr, size_t bytes) {
   if (fault_fires(0)) {
@@ -391,6 +428,7 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
   V *out = (V *)d_out;
   const cpx<R> *usin = (const cpx<R> *)d_usin;
   const bool tilted = p->kind == ODT_KIND_3D_TILTED;
+  StageRange call_range("odt.execute");
   if (p->timing) HIPCHK(hipEventRecord(p->ev_t0, st), "event");
   p->n_rot = 0;
   p->n_rot_k = 0;
@@ -414,6 +452,7 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
       const int tb = overlap ? ci & 1 : 0;
       cpx<R> *Tc = tb ? (cpx<R> *)(ws + p->off_T2) : T;
       hipStream_t rst = overlap ? p->side : st;
+      StageRange chunk_range("odt.filter_chunk");
       if (p->fwd64) {
         cpx<double> *F1d = (cpx<double> *)(ws + p->off_F1);
         HIPCHK((odt::launch_fwd_rows<R, double>(usin, p->weights, F1d,
@@ -485,6 +524,7 @@ int execute_t(odt_plan *p, const void *d_usin, void *d_out, char *ws,
       HIPCHK(hipEventRecord(p->ev_rot[2 * p->n_rot], st), "event");
     }
     p->n_rot_k += !tilted && p->order < 2 ? 1 : (nb + p->RG - 1) / p->RG;
+    StageRange rot_range(tilted ? "odt.affine_accumulate" : "odt.rotate_accumulate");
     if (tilted)
       for (int g0 = 0; g0 < nb; g0 += p->RG) {
         const int gn = std::min(p->RG, nb - g0);
@@ -569,6 +609,7 @@ int upload_fir(odt_plan *p) {
 template <typename R>
 int adjoint_t(odt_plan *p, const void *d_gout, void *d_gusin, char *ws,
               hipStream_t st) {
+  StageRange range("odt.adjoint");
   const Geom &g = p->g;
   cpx<R> *U1 = (cpx<R> *)(ws + p->off_F1);
   cpx<R> *Sp = (cpx<R> *)(ws + p->off_S);
@@ -1148,6 +1189,7 @@ int odt_batch_create(int n, odt_plan *const *plans, const void *const *d_usin,
 
 int odt_batch_launch(odt_batch *b, void *stream) {
   if (!b) return fail(ODT_ERR_INVALID, "null batch");
+  StageRange range("odt.batch_launch");
   HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream), "hipGraphLaunch");
   return ODT_OK;
 }
@@ -1283,6 +1325,7 @@ int odt_sinogram_prepare(int precision, int ndim, int32_t A, int32_t Ny, int32_t
   const int rc = prepare_check(precision, ndim, A, Ny, Nx);
   if (rc != ODT_OK) return rc;
   if (!d_usin || !d_out || !d_ws) return fail(ODT_ERR_INVALID, "null argument");
+  StageRange range("odt.sinogram_prepare");
   size_t need = 0;
   const int H = ndim == 3 ? Ny : 1;
   HIPCHK(precision == ODT_FP64 ? odt::prepare_workspace<double>(ndim, A, H, Nx, rytov, &need)
