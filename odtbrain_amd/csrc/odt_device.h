@@ -348,6 +348,17 @@ __device__ __forceinline__ double bspline_val(double d) {
 // Same weights from the fractional offset t = c - floor(c) (odd orders) or
 // t = c - floor(c + 0.5) (even orders), evaluated in W: the float32 path
 // forms t exactly in float64 and only the polynomials run in float32.
+// x / d for the float32 weights as a product with the rounded reciprocal
+// (an IEEE division costs about seven VALU instructions, and the gather
+// kernels form these weights per voxel and angle); float64 divides.
+template <typename W>
+__device__ __forceinline__ W div_const(W x, double d) {
+  if constexpr (sizeof(W) == 4)
+    return x * (W)(1.0 / d);
+  else
+    return x / (W)d;
+}
+
 template <int ORDER, typename W>
 __device__ __forceinline__ void spline_weights_t(W t, W *w) {
   if (ORDER == 0) {
@@ -362,9 +373,9 @@ __device__ __forceinline__ void spline_weights_t(W t, W *w) {
     w[2] = W(1) - w[0] - w[1];
   } else if (ORDER == 3) {
     const W z = W(1) - t;
-    w[0] = z * z * z / W(6);
-    w[1] = (t * t * (t - W(2)) * W(3) + W(4)) / W(6);
-    w[2] = (z * z * (z - W(2)) * W(3) + W(4)) / W(6);
+    w[0] = div_const(z * z * z, 6.0);
+    w[1] = div_const(t * t * (t - W(2)) * W(3) + W(4), 6.0);
+    w[2] = div_const(z * z * (z - W(2)) * W(3) + W(4), 6.0);
     w[3] = W(1) - w[0] - w[1] - w[2];
   } else if (ORDER == 4) {
     // distances 2+t, 1+t, |t|, 1-t (t in [-0.5, 0.5)); the last weight
@@ -372,11 +383,11 @@ __device__ __forceinline__ void spline_weights_t(W t, W *w) {
     const W d0 = W(2) + t, d1 = W(1) + t, d3 = W(1) - t;
     const W ta = t < W(0) ? -t : t;
     const W e0 = W(2.5) - d0;
-    w[0] = e0 * e0 * e0 * e0 / W(24);
-    w[1] = (W(55) + d1 * (W(20) + d1 * (W(-120) + d1 * (W(80) - W(16) * d1)))) / W(96);
+    w[0] = div_const(e0 * e0 * e0 * e0, 24.0);
+    w[1] = div_const(W(55) + d1 * (W(20) + d1 * (W(-120) + d1 * (W(80) - W(16) * d1))), 96.0);
     const W t2 = ta * ta;
     w[2] = t2 * (t2 * W(0.25) - W(0.625)) + W(115) / W(192);
-    w[3] = (W(55) + d3 * (W(20) + d3 * (W(-120) + d3 * (W(80) - W(16) * d3)))) / W(96);
+    w[3] = div_const(W(55) + d3 * (W(20) + d3 * (W(-120) + d3 * (W(80) - W(16) * d3))), 96.0);
     w[4] = W(1) - w[0] - w[1] - w[2] - w[3];
   } else {  // ORDER == 5, t in [0, 1): distances 2+t, 1+t, t, 1-t, 2-t
     const W d[5] = {W(2) + t, W(1) + t, t, W(1) - t, W(2) - t};
@@ -387,12 +398,12 @@ __device__ __forceinline__ void spline_weights_t(W t, W *w) {
       W v;
       if (j == 0) {  // [2, 3)
         const W e = W(3) - x;
-        v = e * e * e * e * e / W(120);
+        v = div_const(e * e * e * e * e, 120.0);
       } else if (j == 2 || j == 3) {  // [0, 1]
         const W x2 = x * x;
-        v = x2 * (x2 * (W(0.25) - x / W(12)) - W(0.5)) + W(0.55);
+        v = x2 * (x2 * (W(0.25) - div_const(x, 12.0)) - W(0.5)) + W(0.55);
       } else {  // [1, 2]
-        v = x * (x * (x * (x * (x / W(24) - W(0.375)) + W(1.25)) - W(1.75)) +
+        v = x * (x * (x * (x * (div_const(x, 24.0) - W(0.375)) + W(1.25)) - W(1.75)) +
                  W(0.625)) + W(0.425);
       }
       w[j] = v;
