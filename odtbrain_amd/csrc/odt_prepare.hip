@@ -394,9 +394,10 @@ __device__ __forceinline__ void uw_walk2(int p1, int2 pre1, int p2, int2 pre2,
   *s2o = s2;
 }
 
-// On from a root that a merge demoted (its link is in memory: the merging
-// wave's stores complete before each round and batch ends); false if a
-// link still reads as a root.
+// Walks on from a root that a merge demoted to the current root, adding
+// the link offsets to t and reading its size word into g (the merging
+// wave's own links are in memory: its stores complete before each round
+// and batch ends); stops at a link that still reads as a root.
 __device__ __forceinline__ void uw_resolve(int2 *node, const unsigned short *gsz, int &a,
                                            int &t, unsigned &g) {
   for (int d = 0; d < 64 && g == kUwNotRoot; ++d) {
