@@ -104,16 +104,23 @@ def unwrap_2d(img):
             edges.append((rel[i, j] + rel[i + 1, j], i * W + j, (i + 1) * W + j))
     order = sorted(range(len(edges)), key=lambda k: (edges[k][0], k))
     flat = img.ravel()
-    n = H * W
+    inc = merge_groups(H * W, [(edges[k][1], edges[k][2],
+                                _find_wrap(flat[edges[k][1]], flat[edges[k][2]]))
+                               for k in order])
+    return (flat + TWOPI * inc).reshape(H, W)
+
+
+def merge_groups(n, edges):
+    """The merge sequence of the published algorithm: `edges` in sorted
+    order as (p1, p2, e), e the whole-turn step find_wrap(phase[p1],
+    phase[p2]); returns every pixel's whole-turn increment."""
     inc = np.zeros(n, dtype=np.int64)
     group = list(range(n))             # group id (head pixel) of every pixel
     members = {p: [p] for p in range(n)}
-    for k in order:
-        _, p1, p2 = edges[k]
+    for p1, p2, e in edges:
         g1, g2 = group[p1], group[p2]
         if g1 == g2:
             continue
-        e = _find_wrap(flat[p1], flat[p2])
         if len(members[g2]) == 1:        # p2 alone joins p1's group
             inc[p2] = inc[p1] - e
             members[g1].append(p2)
@@ -136,7 +143,7 @@ def unwrap_2d(img):
                 inc[q] += d
                 group[q] = g2
             members[g2].extend(members.pop(g1))
-    return (flat + TWOPI * inc).reshape(H, W)
+    return inc
 
 
 def _unwrapped_phase(phase):
